@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's hot-path interface over the C ABI.
+
+`Context` owns one device context (adga_ctx_create).  Its methods follow the
+reference's loop body (src/main.F90:486-799): `set_local_omega` is the local
+update done when omega changes (506-625), `run_points` the per-point work for a
+batch of q points (635-781), `get_sigma` the accumulated self-energies (841-855).
+`run_loop` drives a whole (omega, q) point list the way main.F90 does and
+returns the same quantities the reference accumulates.  `inverse_matrix` mirrors
+lapack_module's inverse_matrix (src/lapack_module.f90:40-70).
+
+Errors follow the reference's fatal-error behaviour (mpi_stop): a non-zero
+status raises `AdgaError` carrying the status code and the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ADGA_OK, ADGA_PREC_FP64, ADGA_PREC_REFINED, lib
+
+__all__ = ["AdgaError", "Context", "run_loop", "inverse_matrix", "PRECISION"]
+
+PRECISION = {"fp64": ADGA_PREC_FP64, "refined": ADGA_PREC_REFINED}
+
+
+class AdgaError(RuntimeError):
+    def __init__(self, code, msg, status=None):
+        super().__init__(f"adga status {code}: {msg}")
+        self.code = code
+        self.status = status
+
+
+def _zf(a):
+    """Fortran-ordered complex128 copy/view."""
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_lib._dp)
+
+
+class Context:
+    def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None):
+        wl = inp.wl
+        self.n = wl.n
+        self.n2 = wl.n * wl.n
+        self.nf = wl.nf
+        self.iwfmax_small = wl.iwfmax_small
+        self.iwbmax_small = wl.iwbmax_small
+        self.nkp_eom = len(inp.k_data_eom)
+        self.nqp = len(inp.q_data)
+        d = _lib.adga_dims(wl.n, wl.nk[0], wl.nk[1], wl.nk[2], inp.iwmax, wl.iwfmax_small, wl.iwbmax_small,
+                           self.nqp, self.nkp_eom, float(inp.beta), float(inp.mu))
+        self._keep = [_zf(inp.hk), _zf(inp.siw), _zf(inp.giw), _zf(inp.dc), _zf(inp.u), _zf(inp.u_tilde),
+                      _zf(inp.n_dmft)]
+        q = np.ascontiguousarray(inp.q_data, dtype=np.int32)
+        k = np.ascontiguousarray(inp.k_data_eom, dtype=np.int32)
+        h = C.c_void_p()
+        rc = lib.adga_ctx_create(C.byref(d), q.ctypes.data_as(_lib._ip), k.ctypes.data_as(_lib._ip),
+                                 *[_dptr(a) for a in self._keep], C.byref(h))
+        if rc != ADGA_OK:
+            raise AdgaError(rc, "adga_ctx_create failed")
+        self.h = h
+        self._keep = None
+        for key, v in (("batch", batch), ("refine_tol", refine_tol), ("refine_maxit", refine_maxit)):
+            if v is not None:
+                self._check(lib.adga_set_option(self.h, key.encode(), float(v)))
+
+    # -----------------------------------------------------------------
+    def _err(self):
+        buf = C.create_string_buffer(512)
+        lib.adga_last_error(self.h, buf, 512)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc, status=None):
+        if rc != ADGA_OK:
+            raise AdgaError(rc, self._err(), status)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.adga_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -----------------------------------------------------------------
+    def set_local_omega(self, iwb, chi_loc_dens, chi_loc_magn):
+        n2 = self.n2
+        xd, xm = _zf(chi_loc_dens), _zf(chi_loc_magn)
+        N = n2 * self.nf
+        if xd.shape != (N, N) or xm.shape != (N, N):
+            raise ValueError(f"chi_loc must be ({N}, {N})")
+        od = np.zeros((n2, n2), dtype=np.complex128, order="F")
+        om = np.zeros_like(od)
+        bl = np.zeros_like(od)
+        self._check(lib.adga_set_local_omega(self.h, int(iwb), _dptr(xd), _dptr(xm), _dptr(od), _dptr(om), _dptr(bl)))
+        return od, om, bl
+
+    def run_points(self, iq, precision="fp64", raise_on_error=True):
+        """iq: 1-based positions in q_data.  Returns chi_d, chi_m, bubble_nl of shape (n2, n2, npts)
+        and the per-point status."""
+        iq = np.ascontiguousarray(iq, dtype=np.int32)
+        npts = iq.size
+        n2 = self.n2
+        cd = np.zeros((n2, n2, npts), dtype=np.complex128, order="F")
+        cm = np.zeros_like(cd)
+        bn = np.zeros_like(cd)
+        st = np.zeros(npts, dtype=np.int32)
+        rc = lib.adga_run_points(self.h, npts, iq.ctypes.data_as(_lib._ip), PRECISION[precision], _dptr(cd),
+                                 _dptr(cm), _dptr(bn), st.ctypes.data_as(_lib._ip))
+        if raise_on_error:
+            self._check(rc, st)
+        return {"chi_d": cd, "chi_m": cm, "bubble_nl": bn, "status": st, "rc": rc}
+
+    def get_sigma(self):
+        n = self.n
+        snl = np.zeros((n, n, self.nf, self.nkp_eom), dtype=np.complex128, order="F")
+        sdm = np.zeros((n, n, self.nf), dtype=np.complex128, order="F")
+        shf = np.zeros((n, n, self.nkp_eom), dtype=np.complex128, order="F")
+        self._check(lib.adga_get_sigma(self.h, _dptr(snl), _dptr(sdm), _dptr(shf)))
+        return snl, sdm, shf
+
+    def reset_sigma(self):
+        self._check(lib.adga_reset_sigma(self.h))
+
+    def stats(self):
+        out = np.zeros(11)
+        self._check(lib.adga_get_stats(self.h, _dptr(out), 11))
+        keys = ["refine_iters", "refine_resid", "t_local", "t_chi0", "t_gammas", "t_inversion", "t_eta_rest",
+                "t_eom", "t_chi", "t_lu_kernel", "n_lu_launches"]
+        return dict(zip(keys, out.tolist()))
+
+
+def run_loop(ctx: Context, points, chi_loc_fn, precision="fp64"):
+    """Drive `points` = [(iwb, iq0), ...] (iq0 0-based, omega-major like main.F90:381-389) through the
+    device path.  Consecutive points with the same omega form one batch.  Returns the quantities the
+    reference accumulates: per-point chi_d, chi_m, bubble_nl, and sigma_nl, sigma_dmft, chi_loc_d/m and
+    bubble_loc per omega (added when the omega block starts at iq == 1, main.F90:519-577)."""
+    n2 = ctx.n2
+    npts = len(points)
+    out = {"chi_d": np.zeros((n2, n2, npts), dtype=np.complex128, order="F"),
+           "chi_m": np.zeros((n2, n2, npts), dtype=np.complex128, order="F"),
+           "bubble_nl": np.zeros((n2, n2, npts), dtype=np.complex128, order="F"),
+           "status": np.zeros(npts, dtype=np.int32),
+           "chi_loc_d": {}, "chi_loc_m": {}, "bubble_loc": {}}
+    p = 0
+    while p < npts:
+        iwb = points[p][0]
+        e = p
+        while e < npts and points[e][0] == iwb:
+            e += 1
+        ld, lm, bl = ctx.set_local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"))
+        if points[p][1] == 0:
+            out["chi_loc_d"][iwb] = ld
+            out["chi_loc_m"][iwb] = lm
+            out["bubble_loc"][iwb] = bl
+        iq = np.array([q + 1 for (_, q) in points[p:e]], dtype=np.int32)
+        r = ctx.run_points(iq, precision)
+        out["chi_d"][:, :, p:e] = r["chi_d"]
+        out["chi_m"][:, :, p:e] = r["chi_m"]
+        out["bubble_nl"][:, :, p:e] = r["bubble_nl"]
+        out["status"][p:e] = r["status"]
+        p = e
+    out["sigma_nl"], out["sigma_dmft"], out["sigma_hf"] = ctx.get_sigma()
+    return out
+
+
+def inverse_matrix(m):
+    """In-place-style inverse like inverse_matrix_z (src/lapack_module.f90:40-70): returns (m^-1, ierr).
+    Raises AdgaError for a singular matrix (the reference stops with "ERROR in ZGETRF")."""
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError("square matrix expected")
+    ierr = C.c_int(0)
+    rc = lib.adga_inverse_matrix_z(a.shape[0], _dptr(a), C.byref(ierr))
+    if rc != ADGA_OK:
+        raise AdgaError(rc, f"ERROR in ZGETRF (info={ierr.value})" if rc == _lib.ADGA_E_SINGULAR else "inverse failed")
+    return a, ierr.value

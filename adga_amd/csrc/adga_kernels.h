@@ -1,0 +1,40 @@
+// Host-side launchers of the gfx950 kernels (internal to libadga_amd.so).
+#pragma once
+#include "adga_common.h"
+
+namespace adga {
+
+void launch_gtable(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* gt, hipStream_t s);
+void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z64* xm, z64* winv, z64* chi0w,
+                  z64* cd, z64* cm, z64* gam_d, z64* gam_m, z64* opg_d, z64* opg_m, z64* chiloc_d,
+                  z64* chiloc_m, z64* bubloc, hipStream_t s);
+void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s);
+void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64* S, hipStream_t s);
+template <typename T>
+void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
+                  const z64* opgm, const z64* B, cplx<T>* M, z64* gamq, hipStream_t s);
+template <typename T>
+int lu_leaf_width(int N);
+template <typename T>
+bool launch_lu(int nmat, int N, int W, int ldm, cplx<T>* M, int* piv, int* info, hipStream_t s);
+template <typename T>
+bool launch_lu_apply(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* piv, cplx<T>* x,
+                     hipStream_t s);
+template <typename T>
+void launch_extract(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, z64* Y, hipStream_t s);
+void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
+                const z64* opgm, z64* Z, z64* B, hipStream_t s);
+void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
+                     const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s);
+template <typename T>
+void launch_correct(long tot, const z64* R, cplx<T>* x, int stage, z64* Y, hipStream_t s);
+void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
+                    const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
+                    z64* chi_out, z64* bub_out, hipStream_t s);
+void launch_eom(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* u, const z64* ut,
+                const z64* eta, const z64* gamq, z64* mt, const z64* gt, z64* sigma, hipStream_t s);
+void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
+                      const z64* ndmft, z64* sdmft, hipStream_t s);
+void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s);
+
+}  // namespace adga

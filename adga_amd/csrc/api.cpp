@@ -1,0 +1,485 @@
+// C ABI (include/adga_amd.h) over the gfx950 kernels: context, device buffers, batching,
+// refinement loop and error reporting.  No exception crosses the extern "C" boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adga_amd.h"
+#include "adga_kernels.h"
+
+using adga::c32;
+using adga::Geom;
+using adga::z64;
+
+struct adga_ctx {
+  Geom g{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int refine_maxit = 12;
+  double refine_tol = 1e-13;
+  int batch = 256;
+  int ldm = 0;
+  // one-particle data (device)
+  z64 *hk = nullptr, *siw = nullptr, *giw = nullptr, *dc = nullptr, *u = nullptr, *ut = nullptr, *ndmft = nullptr;
+  z64* gt = nullptr;
+  int* keom = nullptr;
+  std::vector<int> q0;  // 0-based k index of each q (host)
+  // current bosonic frequency
+  bool have_omega = false;
+  int iwb = 0;
+  z64 *xd = nullptr, *xm = nullptr, *cd = nullptr, *cm = nullptr, *gamd = nullptr, *gamm = nullptr;
+  z64 *opgd = nullptr, *opgm = nullptr, *winv = nullptr, *chi0w = nullptr;
+  z64 *chiloc_d = nullptr, *chiloc_m = nullptr, *bubloc = nullptr;
+  // batch workspace
+  int cap = 0;
+  z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
+      *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
+  void* M = nullptr;
+  void* xcor = nullptr;
+  double* rel = nullptr;
+  int *piv = nullptr, *info = nullptr, *qidx = nullptr;
+  // accumulated self-energies
+  z64 *sig_nl = nullptr, *sig_dmft = nullptr;
+  double stats[11] = {0};  // see adga_get_stats
+};
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define CK(x)                                   \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) throw HipError{e_, #x}; \
+  } while (0)
+
+template <typename F>
+int guarded(adga_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (const HipError& he) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)he.e, hipGetErrorString(he.e), he.what);
+    if (c) c->err = buf;
+    return he.e == hipErrorOutOfMemory ? ADGA_E_NOMEM : ADGA_E_HIP;
+  } catch (...) {
+    if (c) c->err = "unexpected host exception";
+    return ADGA_E_HIP;
+  }
+}
+
+template <typename T>
+void dalloc(T** p, size_t count) {
+  CK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+void upload(z64* d, const double* h, size_t count, hipStream_t s) {
+  CK(hipMemcpyAsync(d, h, count * sizeof(z64), hipMemcpyHostToDevice, s));
+}
+
+void free_batch(adga_ctx* c) {
+  dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
+  dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->xcor); dfree(c->rel);
+  dfree(c->piv); dfree(c->info); dfree(c->qidx);
+  c->cap = 0;
+}
+
+void ensure_batch(adga_ctx* c, int nb) {
+  if (nb <= c->cap) return;
+  free_batch(c);
+  const Geom& g = c->g;
+  size_t per_pt = (size_t)g.nf * g.n4;
+  size_t pmat = (size_t)g.N * g.n2;
+  dalloc(&c->chi0q, nb * per_pt);
+  dalloc(&c->S, nb * per_pt);
+  dalloc(&c->gamq, nb * pmat);
+  dalloc(&c->Bp, 2 * nb * pmat);
+  dalloc(&c->Y, 2 * nb * pmat);
+  dalloc(&c->Z, 2 * nb * pmat);
+  dalloc(&c->R, 2 * nb * pmat);
+  dalloc(&c->eta, 2 * nb * pmat);
+  dalloc(&c->mt, nb * pmat);
+  dalloc(&c->chi_out, 2 * nb * (size_t)g.n4);
+  dalloc(&c->bub_out, nb * (size_t)g.n4);
+  dalloc((char**)&c->M, 2 * nb * (size_t)g.N * c->ldm * sizeof(z64));
+  dalloc((char**)&c->xcor, 2 * nb * pmat * sizeof(z64));
+  dalloc(&c->rel, 2 * nb);
+  dalloc(&c->piv, 2 * nb * (size_t)g.N);
+  dalloc(&c->info, 2 * nb);
+  dalloc(&c->qidx, nb);
+  c->cap = nb;
+}
+
+int fail(adga_ctx* c, int code, const char* msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom, const double* hk,
+                    const double* siw, const double* giw, const double* dc, const double* u, const double* u_tilde,
+                    const double* n_dmft, adga_ctx** out) {
+  if (!out || !d || !q_data || !k_data_eom || !hk || !siw || !giw || !dc || !u || !u_tilde || !n_dmft)
+    return ADGA_E_ARG;
+  *out = nullptr;
+  if (d->ndim < 1 || d->ndim > 5 || d->nkpx < 1 || d->nkpy < 1 || d->nkpz < 1 || d->iwfmax_small < 1 ||
+      d->iwbmax_small < 0 || d->nqp < 1 || d->nkp_eom < 1 || !(d->beta > 0))
+    return ADGA_E_ARG;
+  if (d->iwmax < d->iwfmax_small + d->iwbmax_small) return ADGA_E_ARG;  // config_module.f90:600-607
+  adga_ctx* c = new (std::nothrow) adga_ctx();
+  if (!c) return ADGA_E_NOMEM;
+  int rc = guarded(c, [&]() -> int {
+    Geom& g = c->g;
+    g.n = d->ndim;
+    g.n2 = g.n * g.n;
+    g.n4 = g.n2 * g.n2;
+    g.iwf = d->iwfmax_small;
+    g.nf = 2 * g.iwf;
+    g.iwbmax = d->iwbmax_small;
+    g.N = g.n2 * g.nf;
+    g.nkx = d->nkpx; g.nky = d->nkpy; g.nkz = d->nkpz;
+    g.nkp = g.nkx * g.nky * g.nkz;
+    g.glo = -g.iwf - g.iwbmax;
+    g.nw = g.nf + 2 * g.iwbmax;
+    g.iwmax = d->iwmax;
+    g.nqp = d->nqp;
+    g.nkp_eom = d->nkp_eom;
+    g.beta = d->beta;
+    g.mu = d->mu;
+    int W = g.N + g.n2;
+    c->ldm = (W + 15) / 16 * 16;
+    c->q0.resize(g.nqp);
+    for (int i = 0; i < g.nqp; ++i) {
+      if (q_data[i] < 1 || q_data[i] > g.nkp) throw HipError{hipErrorInvalidValue, "q_data out of range"};
+      c->q0[i] = q_data[i] - 1;
+    }
+    for (int i = 0; i < g.nkp_eom; ++i)
+      if (k_data_eom[i] < 1 || k_data_eom[i] > g.nkp) throw HipError{hipErrorInvalidValue, "k_data_eom out of range"};
+    CK(hipStreamCreate(&c->stream));
+    hipStream_t s = c->stream;
+    dalloc(&c->hk, (size_t)g.n2 * g.nkp);
+    dalloc(&c->siw, (size_t)2 * g.iwmax * g.n);
+    dalloc(&c->giw, (size_t)2 * g.iwmax * g.n);
+    dalloc(&c->dc, (size_t)2 * g.n);
+    dalloc(&c->u, (size_t)g.n4);
+    dalloc(&c->ut, (size_t)g.n4);
+    dalloc(&c->ndmft, (size_t)g.n);
+    dalloc(&c->gt, (size_t)g.nw * g.nkp * g.n2);
+    dalloc(&c->keom, (size_t)g.nkp_eom);
+    upload(c->hk, hk, (size_t)g.n2 * g.nkp, s);
+    upload(c->siw, siw, (size_t)2 * g.iwmax * g.n, s);
+    upload(c->giw, giw, (size_t)2 * g.iwmax * g.n, s);
+    upload(c->dc, dc, (size_t)2 * g.n, s);
+    upload(c->u, u, (size_t)g.n4, s);
+    upload(c->ut, u_tilde, (size_t)g.n4, s);
+    upload(c->ndmft, n_dmft, (size_t)g.n, s);
+    CK(hipMemcpyAsync(c->keom, k_data_eom, sizeof(int) * g.nkp_eom, hipMemcpyHostToDevice, s));
+    adga::launch_gtable(g, c->hk, c->siw, c->dc, c->gt, s);
+    CK(hipGetLastError());
+    size_t NN = (size_t)g.N * g.N, pm = (size_t)g.n2 * g.N;
+    dalloc(&c->xd, NN); dalloc(&c->xm, NN); dalloc(&c->cd, NN); dalloc(&c->cm, NN);
+    dalloc(&c->gamd, pm); dalloc(&c->gamm, pm); dalloc(&c->opgd, pm); dalloc(&c->opgm, pm);
+    dalloc(&c->winv, (size_t)g.n2 * g.nf); dalloc(&c->chi0w, (size_t)g.n2 * g.nf);
+    dalloc(&c->chiloc_d, (size_t)g.n4); dalloc(&c->chiloc_m, (size_t)g.n4); dalloc(&c->bubloc, (size_t)g.n4);
+    dalloc(&c->sig_nl, (size_t)g.n2 * g.nf * g.nkp_eom);
+    dalloc(&c->sig_dmft, (size_t)g.n2 * g.nf);
+    CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, s));
+    CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, s));
+    CK(hipStreamSynchronize(s));
+    return ADGA_OK;
+  });
+  if (rc != ADGA_OK) {
+    std::string msg = c->err;
+    adga_ctx_destroy(c);
+    (void)msg;
+    return rc;
+  }
+  *out = c;
+  return ADGA_OK;
+}
+
+int adga_set_option(adga_ctx* c, const char* key, double v) {
+  if (!c || !key) return ADGA_E_ARG;
+  if (!strcmp(key, "batch")) {
+    if (!(v >= 1 && v <= 1 << 20)) return fail(c, ADGA_E_ARG, "batch must be in [1, 2^20]");
+    c->batch = (int)v;
+    return ADGA_OK;
+  }
+  if (!strcmp(key, "refine_tol")) {
+    if (!(v > 0 && v < 1)) return fail(c, ADGA_E_ARG, "refine_tol must be in (0, 1)");
+    c->refine_tol = v;
+    return ADGA_OK;
+  }
+  if (!strcmp(key, "refine_maxit")) {
+    if (!(v >= 1 && v <= 100)) return fail(c, ADGA_E_ARG, "refine_maxit must be in [1, 100]");
+    c->refine_maxit = (int)v;
+    return ADGA_OK;
+  }
+  return fail(c, ADGA_E_ARG, "unknown option");
+}
+
+int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* xm, double* chid_out,
+                         double* chim_out, double* bubloc_out) {
+  if (!c || !xd || !xm) return ADGA_E_ARG;
+  if (iwb < -c->g.iwbmax || iwb > c->g.iwbmax) return fail(c, ADGA_E_ARG, "iwb outside [-iwbmax_small, iwbmax_small]");
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    hipStream_t s = c->stream;
+    size_t NN = (size_t)g.N * g.N;
+    upload(c->xd, xd, NN, s);
+    upload(c->xm, xm, NN, s);
+    adga::launch_local(g, iwb, c->giw, c->xd, c->xm, c->winv, c->chi0w, c->cd, c->cm, c->gamd, c->gamm, c->opgd,
+                       c->opgm, c->chiloc_d, c->chiloc_m, c->bubloc, s);
+    CK(hipGetLastError());
+    if (chid_out) CK(hipMemcpyAsync(chid_out, c->chiloc_d, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
+    if (chim_out) CK(hipMemcpyAsync(chim_out, c->chiloc_m, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
+    if (bubloc_out) CK(hipMemcpyAsync(bubloc_out, c->bubloc, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    c->iwb = iwb;
+    c->have_omega = true;
+    return ADGA_OK;
+  });
+}
+
+int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double* chid, double* chim,
+                    double* bubnl, int* status) {
+  if (!c || npts < 0 || (npts > 0 && !iq)) return ADGA_E_ARG;
+  if (precision != ADGA_PREC_FP64 && precision != ADGA_PREC_REFINED) return fail(c, ADGA_E_ARG, "bad precision");
+  if (!c->have_omega) return fail(c, ADGA_E_STATE, "adga_set_local_omega must be called before adga_run_points");
+  for (int p = 0; p < npts; ++p)
+    if (iq[p] < 1 || iq[p] > c->g.nqp) return fail(c, ADGA_E_ARG, "iq outside [1, nqp]");
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    hipStream_t s = c->stream;
+    const int n2 = g.n2, W = g.N + g.n2;
+    std::fill(c->stats, c->stats + 11, 0.0);
+    int worst = ADGA_OK;
+    hipEvent_t ev[8], lu0, lu1;
+    CK(hipEventCreate(&lu0));
+    CK(hipEventCreate(&lu1));
+    for (auto& e : ev) CK(hipEventCreate(&e));
+    std::vector<int> qh, infoh;
+    std::vector<double> relh;
+    std::vector<z64> chih, bubh;
+    for (int p0 = 0; p0 < npts; p0 += c->batch) {
+      const int nb = std::min(c->batch, npts - p0);
+      ensure_batch(c, nb);
+      qh.resize(nb);
+      for (int b = 0; b < nb; ++b) qh[b] = c->q0[iq[p0 + b] - 1];
+      CK(hipMemcpyAsync(c->qidx, qh.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+      const int nmat = 2 * nb;
+      const long tot = (long)nmat * g.N * n2;
+      CK(hipEventRecord(ev[0], s));
+      // local EOM for points at the Gamma point (main.F90:635-639)
+      for (int b = 0; b < nb; ++b)
+        if (iq[p0 + b] == 1)
+          adga::launch_eom_local(g, c->iwb, c->iwb == 0, c->u, c->gamd, c->giw, c->ndmft, c->sig_dmft, s);
+      CK(hipEventRecord(ev[1], s));
+      adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->chi0q, s);
+      adga::launch_smat(g, nb, c->chi0q, c->winv, c->S, s);
+      adga::launch_rhs(g, nb, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Z, c->Bp, s);
+      CK(hipEventRecord(ev[2], s));
+      int iters = 0;
+      double maxrel = 0.0;
+      bool lu_ok;
+      if (precision == ADGA_PREC_FP64) {
+        adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (z64*)c->M, c->gamq, s);
+        CK(hipEventRecord(ev[3], s));
+        CK(hipEventRecord(lu0, s));
+        lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, (z64*)c->M, c->piv, c->info, s);
+        CK(hipEventRecord(lu1, s));
+        if (lu_ok) {
+          adga::launch_extract<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->Y, s);
+          // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
+          relh.resize(nmat);
+          adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
+          CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
+          CK(hipStreamSynchronize(s));
+          for (double r : relh)
+            if (std::isfinite(r)) maxrel = std::max(maxrel, r);
+        }
+      } else {
+        adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (c32*)c->M, c->gamq, s);
+        CK(hipEventRecord(ev[3], s));
+        CK(hipEventRecord(lu0, s));
+        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, (c32*)c->M, c->piv, c->info, s);
+        CK(hipEventRecord(lu1, s));
+        if (lu_ok) adga::launch_extract<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->Y, s);
+        relh.resize(nmat);
+        for (int it = 0; lu_ok; ++it) {
+          adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
+          CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
+          CK(hipStreamSynchronize(s));
+          maxrel = 0.0;
+          bool bad = false;
+          for (double r : relh) {
+            if (!std::isfinite(r)) bad = true;
+            else maxrel = std::max(maxrel, r);
+          }
+          if (bad || maxrel <= c->refine_tol || it >= c->refine_maxit) break;
+          adga::launch_correct<float>(tot, c->R, (c32*)c->xcor, 0, c->Y, s);
+          adga::launch_lu_apply<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (c32*)c->xcor, s);
+          adga::launch_correct<float>(tot, c->R, (c32*)c->xcor, 1, c->Y, s);
+          iters = it + 1;
+        }
+      }
+      if (!lu_ok) return fail(c, ADGA_E_ARG, "BSE size outside the supported range of the LU kernel");
+      CK(hipGetLastError());
+      CK(hipEventRecord(ev[4], s));
+      adga::launch_eta_chi(g, nb, c->Y, c->opgd, c->opgm, c->eta, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc,
+                           c->chi_out, c->bub_out, s);
+      CK(hipEventRecord(ev[5], s));
+      adga::launch_eom(g, c->iwb, nb, c->qidx, c->keom, c->u, c->ut, c->eta, c->gamq, c->mt, c->gt, c->sig_nl, s);
+      CK(hipEventRecord(ev[6], s));
+      CK(hipGetLastError());
+      chih.resize((size_t)2 * nb * g.n4);
+      bubh.resize((size_t)nb * g.n4);
+      infoh.resize(nmat);
+      CK(hipMemcpyAsync(chih.data(), c->chi_out, sizeof(z64) * chih.size(), hipMemcpyDeviceToHost, s));
+      CK(hipMemcpyAsync(bubh.data(), c->bub_out, sizeof(z64) * bubh.size(), hipMemcpyDeviceToHost, s));
+      CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
+      CK(hipStreamSynchronize(s));
+      float ms[6];
+      for (int k = 0; k < 6; ++k) CK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+      c->stats[2 + 5] += ms[0];            // EOM (local part)
+      c->stats[2 + 1] += ms[1];            // Chi0
+      c->stats[2 + 2] += ms[2];            // Gammas (BSE build)
+      c->stats[2 + 3] += ms[3];            // Inversion (LU, solve, refinement)
+      c->stats[2 + 6] += ms[4];            // Chi (eta + contraction)
+      c->stats[2 + 5] += ms[5];            // EOM
+      float lums = 0.f;
+      CK(hipEventElapsedTime(&lums, lu0, lu1));
+      c->stats[9] += lums;                 // the batched LU kernel alone
+      c->stats[10] += 1;                   // its launches
+      c->stats[0] = std::max(c->stats[0], (double)iters);
+      c->stats[1] = std::max(c->stats[1], maxrel);
+      for (int b = 0; b < nb; ++b) {
+        const int p = p0 + b;
+        int st = ADGA_OK;
+        if (infoh[2 * b] != 0 || infoh[2 * b + 1] != 0) st = ADGA_E_SINGULAR;
+        if (st == ADGA_OK) {
+          double r = std::max(relh[2 * b], relh[2 * b + 1]);
+          if (!std::isfinite(r)) st = ADGA_E_NONFINITE;
+          else if (precision == ADGA_PREC_REFINED && r > c->refine_tol) st = ADGA_E_NOCONV;
+        }
+        if (status) status[p] = st;
+        if (st != ADGA_OK && worst == ADGA_OK) worst = st;
+        if (chid) memcpy(chid + 2 * (size_t)p * g.n4, &chih[(size_t)(2 * b) * g.n4], sizeof(z64) * g.n4);
+        if (chim) memcpy(chim + 2 * (size_t)p * g.n4, &chih[(size_t)(2 * b + 1) * g.n4], sizeof(z64) * g.n4);
+        if (bubnl) memcpy(bubnl + 2 * (size_t)p * g.n4, &bubh[(size_t)b * g.n4], sizeof(z64) * g.n4);
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(lu0);
+    (void)hipEventDestroy(lu1);
+    if (worst == ADGA_E_SINGULAR) c->err = "singular BSE matrix (zero pivot in the LU)";
+    else if (worst == ADGA_E_NOCONV) c->err = "iterative refinement did not reach the tolerance";
+    else if (worst == ADGA_E_NONFINITE) c->err = "non-finite value in a BSE solve";
+    return worst;
+  });
+}
+
+int adga_get_sigma(adga_ctx* c, double* sig_nl, double* sig_dmft, double* sig_hf) {
+  if (!c) return ADGA_E_ARG;
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    hipStream_t s = c->stream;
+    adga::launch_fill_neg(g, g.nkp_eom, c->sig_nl, s);
+    CK(hipGetLastError());
+    if (sig_nl) CK(hipMemcpyAsync(sig_nl, c->sig_nl, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, hipMemcpyDeviceToHost, s));
+    if (sig_dmft) CK(hipMemcpyAsync(sig_dmft, c->sig_dmft, sizeof(z64) * g.n2 * g.nf, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    if (sig_hf) memset(sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom);
+    return ADGA_OK;
+  });
+}
+
+int adga_reset_sigma(adga_ctx* c) {
+  if (!c) return ADGA_E_ARG;
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, c->stream));
+    CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ADGA_OK;
+  });
+}
+
+int adga_get_stats(adga_ctx* c, double* out, int nout) {
+  if (!c || !out || nout < 0) return ADGA_E_ARG;
+  for (int i = 0; i < nout && i < 11; ++i) out[i] = c->stats[i];
+  return ADGA_OK;
+}
+
+int adga_last_error(adga_ctx* c, char* buf, int len) {
+  if (!c || !buf || len <= 0) return ADGA_E_ARG;
+  snprintf(buf, (size_t)len, "%s", c->err.c_str());
+  return ADGA_OK;
+}
+
+int adga_ctx_destroy(adga_ctx* c) {
+  if (!c) return ADGA_OK;
+  free_batch(c);
+  dfree(c->hk); dfree(c->siw); dfree(c->giw); dfree(c->dc); dfree(c->u); dfree(c->ut); dfree(c->ndmft);
+  dfree(c->gt); dfree(c->keom);
+  dfree(c->xd); dfree(c->xm); dfree(c->cd); dfree(c->cm); dfree(c->gamd); dfree(c->gamm); dfree(c->opgd);
+  dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
+  dfree(c->sig_nl); dfree(c->sig_dmft);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ADGA_OK;
+}
+
+int adga_inverse_matrix_z(int n, double* m, int* ierr) {
+  if (n < 1 || !m || !ierr) return ADGA_E_ARG;
+  *ierr = 0;
+  return guarded(nullptr, [&]() -> int {
+    const int W = 2 * n, ldm = (W + 15) / 16 * 16;
+    std::vector<z64> h((size_t)n * ldm);
+    const z64* src = reinterpret_cast<const z64*>(m);
+    // row r of the row-major buffer = column r of the Fortran matrix, i.e. the factored
+    // matrix is M^T; solving M^T Y = 1 gives Y = M^-T, whose rows are the columns of M^-1.
+    for (int r = 0; r < n; ++r) {
+      for (int cc = 0; cc < n; ++cc) h[(size_t)r * ldm + cc] = src[(size_t)r * n + cc];
+      for (int cc = n; cc < ldm; ++cc) h[(size_t)r * ldm + cc] = adga::mk(cc - n == r ? 1.0 : 0.0, 0.0);
+    }
+    z64* dM = nullptr;
+    int *dpiv = nullptr, *dinfo = nullptr;
+    dalloc(&dM, h.size());
+    dalloc(&dpiv, (size_t)n);
+    dalloc(&dinfo, 1);
+    CK(hipMemcpy(dM, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
+    bool ok = adga::launch_lu<double>(1, n, W, ldm, dM, dpiv, dinfo, nullptr);
+    if (ok) {
+      CK(hipGetLastError());
+      CK(hipMemcpy(h.data(), dM, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
+      CK(hipMemcpy(ierr, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    dfree(dM);
+    dfree(dpiv);
+    dfree(dinfo);
+    if (!ok) return ADGA_E_ARG;
+    if (*ierr != 0) return ADGA_E_SINGULAR;
+    z64* dst = reinterpret_cast<z64*>(m);
+    for (int r = 0; r < n; ++r)
+      for (int cc = 0; cc < n; ++cc) dst[(size_t)r * n + cc] = h[(size_t)r * ldm + n + cc];
+    return ADGA_OK;
+  });
+}
+
+}  // extern "C"

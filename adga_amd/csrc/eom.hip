@@ -1,0 +1,194 @@
+// A8 (eta), A9 (chi contraction), A10 (dynamic EOM), A11 (local/static EOM) on gfx950.
+#include "adga_kernels.h"
+
+namespace adga {
+
+// eta_m(s, x) = Y[m][x][s]: Y = eta^T solves A^T eta^T = B (main.F90:740-757).  Output (n2, N) Fortran.
+__global__ void k_eta(Geom g, int nmat, const z64* __restrict__ Y, const z64* __restrict__ opg_d,
+                      const z64* __restrict__ opg_m, z64* __restrict__ eta) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long per = (long)g.n2 * g.N;
+  if (idx >= per * nmat) return;
+  int m = (int)(idx / per);
+  long e = idx % per;
+  int s = (int)(e % g.n2);
+  long x = e / g.n2;
+  eta[idx] = Y[((long)m * g.N + x) * g.n2 + s];
+}
+
+// chi_ch(q) (non-local part) = beta^-2 gamma^w_ch . (chi0^q - chi0^w) + beta^-2 eta_ch . chi0^q, where
+// (X . Y)(s, s2) = sum_{d, i3} X(s, d*n2 + i3) Y(i3, s2, d)  (susc_module.f90:34-56, main.F90:666-769);
+// bubble_nl(q) = sum_d chi0^q / beta^2 - sum_d chi0^w / beta^2  (main.F90:654-663).
+__global__ void k_chi(Geom g, int npts, const z64* __restrict__ chi0q, const z64* __restrict__ chi0w,
+                      const z64* __restrict__ gam_d, const z64* __restrict__ gam_m, const z64* __restrict__ eta,
+                      const z64* __restrict__ bubloc, z64* __restrict__ chi_out, z64* __restrict__ bub_out) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)npts * 2 * g.n4) return;
+  int e = (int)(idx % g.n4);
+  int ch = (int)((idx / g.n4) % 2);
+  int b = (int)(idx / (2 * g.n4));
+  int s = e % g.n2, s2 = e / g.n2;
+  const z64* gam = ch == 0 ? gam_d : gam_m;
+  const z64* et = eta + (long)(2 * b + ch) * g.n2 * g.N;
+  const z64* cq = chi0q + (long)b * g.nf * g.n4;
+  z64 a1 = mk(0.0, 0.0), a2 = mk(0.0, 0.0), bs = mk(0.0, 0.0);
+  for (int d = 0; d < g.nf; ++d) {
+    for (int i3 = 0; i3 < g.n2; ++i3) {
+      long x = (long)d * g.n2 + i3;
+      z64 c = cq[(long)d * g.n4 + i3 + s2 * g.n2];
+      z64 cnl = c;
+      if (i3 == s2) cnl = csub(c, chi0w[s2 + d * g.n2]);
+      a1 = cfma(gam[s + x * g.n2], cnl, a1);
+      a2 = cfma(et[s + x * g.n2], c, a2);
+    }
+    if (ch == 0) bs = cadd(bs, cq[(long)d * g.n4 + s + s2 * g.n2]);
+  }
+  double ib2 = 1.0 / (g.beta * g.beta);
+  z64 v = cadd(cscale(a1, ib2), cscale(a2, ib2));
+  chi_out[((long)b * 2 + ch) * g.n4 + e] = v;
+  if (ch == 0) bub_out[(long)b * g.n4 + e] = csub(cscale(bs, ib2), bubloc[e]);
+}
+
+// m_tot = -(U - U~/2) eta_d /(beta nqp) + 1.5 U~ eta_m /(beta nqp) + U gamma^q_d /(beta nqp)
+// (eom_module.f90:187-201; no V(q)).  Output (n2, N) Fortran per point.
+__global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* __restrict__ ut,
+                       const z64* __restrict__ eta, const z64* __restrict__ gamq, z64* __restrict__ mt) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long per = (long)g.n2 * g.N;
+  if (idx >= per * npts) return;
+  int b = (int)(idx / per);
+  long e = idx % per;
+  int s = (int)(e % g.n2);
+  long x = e / g.n2;
+  const z64* ed = eta + (long)(2 * b) * per;
+  const z64* em = eta + (long)(2 * b + 1) * per;
+  const z64* gq = gamq + (long)b * per;
+  z64 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0);
+  for (int a = 0; a < g.n2; ++a) {
+    z64 uu = u[s + a * g.n2], uw = ut[s + a * g.n2];
+    z64 uwork = mk(uu.re - 0.5 * uw.re, uu.im - 0.5 * uw.im);
+    t1 = cfma(uwork, ed[a + x * g.n2], t1);
+    t2 = cfma(uw, em[a + x * g.n2], t2);
+    t3 = cfma(uu, gq[a + x * g.n2], t3);
+  }
+  double a1 = -1.0 / g.beta / g.nqp, a2 = 1.5 / g.beta / g.nqp, a3 = 1.0 / g.beta / g.nqp;
+  z64 m = cscale(t1, a1);
+  m = cadd(cscale(t2, a2), m);
+  m = cadd(cscale(t3, a3), m);
+  mt[idx] = m;
+}
+
+// Sigma_nl(i, l, nu, k) += sum_b sum_{j,k'} m_b(i*n + j, n2*(iwf + nu) + l*n + k') G(k - q_b, nu - omega)_{jk'}
+// for nu = 0 .. iwf-1 (eom_module.f90:211-227).  One thread per (i, l, nu, k); the batch is summed
+// in a fixed order, so the accumulation is deterministic.
+__global__ void __launch_bounds__(256) k_eom(Geom g, int iwb, int npts, const int* __restrict__ qidx,
+                                             const int* __restrict__ keom, const z64* __restrict__ mt,
+                                             const z64* __restrict__ gt, z64* __restrict__ sigma) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int half = g.iwf;
+  if (idx >= (long)g.n2 * half * g.nkp_eom) return;
+  int il = (int)(idx % g.n2);
+  int nu = (int)((idx / g.n2) % half);
+  int kk = (int)(idx / ((long)g.n2 * half));
+  int i = il % g.n, l = il / g.n;
+  int k = keom[kk] - 1;
+  const z64* gw = gt + (long)(nu - iwb - g.glo) * g.nkp * g.n2;
+  long col0 = (long)g.n2 * (g.iwf + nu) + l * g.n;
+  long per = (long)g.n2 * g.N;
+  z64 acc = mk(0.0, 0.0);
+  for (int b = 0; b < npts; ++b) {
+    int kq = k_minus_q(k, qidx[b], g.nkx, g.nky, g.nkz);
+    const z64* gr = gw + (long)kq * g.n2;
+    const z64* m = mt + (long)b * per;
+    for (int j = 0; j < g.n; ++j)
+      for (int kp = 0; kp < g.n; ++kp) acc = cfma(m[(i * g.n + j) + (col0 + kp) * g.n2], gr[j * g.n + kp], acc);
+  }
+  long o = i + (long)l * g.n + (long)(g.iwf + nu) * g.n2 + (long)kk * g.n2 * g.nf;
+  sigma[o] = cadd(sigma[o], acc);
+}
+
+// Sigma(i, l, -nu-1) = conj(Sigma(l, i, nu)) for nu >= 0 (eom_module.f90:157, 228); nk slices.
+__global__ void k_fill_neg(Geom g, int nk, z64* __restrict__ sigma) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)g.n2 * g.iwf * nk) return;
+  int il = (int)(idx % g.n2);
+  int nu = (int)((idx / g.n2) % g.iwf);
+  long kk = idx / ((long)g.n2 * g.iwf);
+  int i = il % g.n, l = il / g.n;
+  long base = kk * g.n2 * g.nf;
+  z64 v = sigma[base + l + (long)i * g.n + (long)(g.iwf + nu) * g.n2];
+  sigma[base + i + (long)l * g.n + (long)(g.iwf - nu - 1) * g.n2] = cconj(v);
+}
+
+// Local EOM with the DMFT g: Sigma_dmft(i, l, nu) += sum_k m(i*n + k, n2*(iwf+nu) + l*n + k) g_k(nu - omega),
+// m = -(1/beta) U gamma^w_d (eom_module.f90:137-158, default branch).
+__global__ void k_eom_dmft(Geom g, int iwb, const z64* __restrict__ u, const z64* __restrict__ gam_d,
+                           const z64* __restrict__ giw, z64* __restrict__ sdmft) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.n2 * g.iwf) return;
+  int il = idx % g.n2, nu = idx / g.n2;
+  int i = il % g.n, l = il / g.n;
+  long col0 = (long)g.n2 * (g.iwf + nu) + l * g.n;
+  z64 acc = mk(0.0, 0.0);
+  for (int k = 0; k < g.n; ++k) {
+    int a = i * g.n + k;
+    z64 t = mk(0.0, 0.0);
+    for (int ap = 0; ap < g.n2; ++ap) t = cfma(u[a + ap * g.n2], gam_d[ap + (col0 + k) * g.n2], t);
+    z64 m = cscale(t, -1.0 / g.beta);
+    z64 gl = giw[(nu - iwb + g.iwmax) + (long)k * 2 * g.iwmax];
+    acc = cfma(m, gl, acc);
+  }
+  long o = i + (long)l * g.n + (long)(g.iwf + nu) * g.n2;
+  sdmft[o] = cadd(sdmft[o], acc);
+}
+
+// Local Hartree term sum_j (2 u({ii},{jj}) - u({ji},{ji})) n_j added to every frequency
+// (eom_module.f90:58-70).
+__global__ void k_eom_static(Geom g, const z64* __restrict__ u, const z64* __restrict__ ndmft,
+                             z64* __restrict__ sdmft) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.n * g.nf) return;
+  int i = idx % g.n, nu = idx / g.n;
+  z64 acc = mk(0.0, 0.0);
+  for (int j = 0; j < g.n; ++j) {
+    z64 a = u[(i * g.n + i) + (long)(j * g.n + j) * g.n2];
+    z64 b = u[(i * g.n + j) + (long)(i * g.n + j) * g.n2];
+    acc = cfma(mk(2.0 * a.re - b.re, 2.0 * a.im - b.im), ndmft[j], acc);
+  }
+  long o = i + (long)i * g.n + (long)nu * g.n2;
+  sdmft[o] = cadd(sdmft[o], acc);
+}
+
+void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
+                    const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
+                    z64* chi_out, z64* bub_out, hipStream_t s) {
+  long tot = (long)2 * npts * g.n2 * g.N;
+  hipLaunchKernelGGL(k_eta, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, 2 * npts, Y, opgd, opgm, eta);
+  long t2 = (long)npts * 2 * g.n4;
+  hipLaunchKernelGGL(k_chi, dim3((unsigned)((t2 + 127) / 128)), dim3(128), 0, s, g, npts, chi0q, chi0w, gamd, gamm,
+                     eta, bubloc, chi_out, bub_out);
+}
+
+void launch_eom(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* u, const z64* ut,
+                const z64* eta, const z64* gamq, z64* mt, const z64* gt, z64* sigma, hipStream_t s) {
+  long tot = (long)npts * g.n2 * g.N;
+  hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt);
+  long te = (long)g.n2 * g.iwf * g.nkp_eom;
+  hipLaunchKernelGGL(k_eom, dim3((unsigned)((te + 255) / 256)), dim3(256), 0, s, g, iwb, npts, qidx, keom, mt, gt,
+                     sigma);
+}
+
+void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
+                      const z64* ndmft, z64* sdmft, hipStream_t s) {
+  if (do_static)
+    hipLaunchKernelGGL(k_eom_static, dim3((g.n * g.nf + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
+  hipLaunchKernelGGL(k_eom_dmft, dim3((g.n2 * g.iwf + 63) / 64), dim3(64), 0, s, g, iwb, u, gamd, giw, sdmft);
+  hipLaunchKernelGGL(k_fill_neg, dim3((g.n2 * g.iwf + 63) / 64), dim3(64), 0, s, g, 1, sdmft);
+}
+
+void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s) {
+  long tot = (long)g.n2 * g.iwf * nk;
+  hipLaunchKernelGGL(k_fill_neg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nk, sigma);
+}
+
+}  // namespace adga

@@ -1,0 +1,332 @@
+"""Seeded synthetic inputs for the per-(q,omega) hot path.
+
+The reference reads its inputs from HDF5 files (H(k), Sigma_DMFT, G_DMFT, the
+local two-particle susceptibility) and a config file.  None of the SrVO3 input
+files ship with the reference, so every workload here is synthetic and fully
+determined by a `Workload` description and its seed:
+
+* H(k): an analytic tight-binding model on the k-grid ordered like
+  `k_index_3` (kq_tools.f90:119-125: ik = 1 + kz + ky*nkz + kx*nky*nkz).
+* Sigma_DMFT(nu): an orbital-diagonal Hubbard-I-like form
+  h_a + s_a^2/(i nu - e_a), which satisfies Sigma(-nu-1) = Sigma(nu)^*.
+* g(nu): the k-summed diagonal of G, computed like `get_giw`
+  (one_particle_quant_module.f90:40-78).
+* n_dmft: like `get_ndmft` (one_particle_quant_module.f90:295-313).
+* U, U~: Kanamori matrices in compound index, like `create_u`
+  (interaction_module.f90:182-291) for one correlated atom.
+* chi_loc_r(omega): what `read_vertex` (hdf5_module.f90:629-773) returns, i.e.
+  the local bubble -beta g_i(nu) g_j(nu-omega) on the diagonal plus a seeded
+  Hermitian complex noise 1e-3 (X + X^H)/2.
+
+All arrays are complex128 and carry the reference's Fortran shapes and index
+meaning (first index fastest): hk(n,n,nkp), siw/giw(2*iwmax, n) with frequency
+offset iwmax, u(n^2,n^2), chi_loc(N,N).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+__all__ = ["Workload", "Inputs", "WORKLOADS", "workload", "build_inputs", "chi_loc",
+           "kanamori_u", "generate_q_vol", "k_minus_q"]
+
+
+@dataclasses.dataclass(frozen=True)
+class Workload:
+    name: str
+    n: int                      # orbitals (ndim)
+    nk: tuple                   # (nkx, nky, nkz)
+    iwfmax_small: int           # Nf/2
+    iwbmax_small: int           # N_omega = 2*iwbmax_small + 1
+    beta: float
+    U: float
+    J: float
+    model: str                  # "sc1", "t2g", "rand5"
+    t: float = 0.0
+    tp: float = 0.0
+    seed: int = 20261202
+    nq: tuple | None = None     # q grid (defaults to the k grid)
+    iwmax: int | None = None    # 1P frequencies (defaults to 4*(Nf/2+Nw/2))
+    noise: float = 1e-3
+
+    @property
+    def nkp(self):
+        return self.nk[0] * self.nk[1] * self.nk[2]
+
+    @property
+    def qgrid(self):
+        return self.nq if self.nq is not None else self.nk
+
+    @property
+    def nqp(self):
+        q = self.qgrid
+        return q[0] * q[1] * q[2]
+
+    @property
+    def nf(self):
+        return 2 * self.iwfmax_small
+
+    @property
+    def maxdim(self):
+        return self.n * self.n * self.nf
+
+    @property
+    def n_iwmax(self):
+        if self.iwmax is not None:
+            return self.iwmax
+        return 4 * (self.iwfmax_small + self.iwbmax_small)
+
+
+WORKLOADS = {
+    # 1-orbital Hubbard model, simple cubic, NN t=0.25, U=2, beta=10, half filling.
+    "sc1_k8": Workload("sc1_k8", 1, (8, 8, 8), 20, 5, 10.0, 2.0, 0.0, "sc1", t=0.25, seed=20261202),
+    # 3-orbital t2g cubic model, NN t=-0.26, NNN t'=-0.08, Kanamori U=5, J=0.75.
+    "t2g_k20": Workload("t2g_k20", 3, (20, 20, 20), 60, 15, 10.0, 5.0, 0.75, "t2g",
+                        t=-0.26, tp=-0.08, seed=20261203),
+    # 5-orbital d-shell, random symmetric hopping to 3rd neighbours (scale 0.3).
+    "d5_k16": Workload("d5_k16", 5, (16, 16, 16), 80, 20, 10.0, 4.0, 0.7, "rand5",
+                       t=0.3, seed=20261205),
+    # 3-orbital t2g model on a 12^3 grid with 2x40 x 21 frequencies.
+    "t2g_k12": Workload("t2g_k12", 3, (12, 12, 12), 40, 10, 10.0, 5.0, 0.75, "t2g",
+                        t=-0.26, tp=-0.08, seed=20261206),
+    # Small cases used by the parity tests (each finishes in seconds on the CPU).
+    "t2g_k4": Workload("t2g_k4", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
+                       t=-0.26, tp=-0.08, seed=11),
+    "t2g_k6": Workload("t2g_k6", 3, (6, 6, 6), 12, 3, 10.0, 5.0, 0.75, "t2g",
+                       t=-0.26, tp=-0.08, seed=12),
+    "two_k4": Workload("two_k4", 2, (4, 4, 2), 4, 2, 8.0, 3.0, 0.5, "rand5",
+                       t=0.3, seed=13),
+    "d5_k4": Workload("d5_k4", 5, (4, 4, 4), 4, 1, 10.0, 4.0, 0.7, "rand5",
+                      t=0.3, seed=14),
+    "sc1_k4": Workload("sc1_k4", 1, (4, 4, 4), 8, 2, 10.0, 2.0, 0.0, "sc1", t=0.25, seed=15),
+}
+
+
+def workload(name: str, **overrides) -> Workload:
+    wl = WORKLOADS[name]
+    if overrides:
+        wl = dataclasses.replace(wl, **overrides)
+    return wl
+
+
+@dataclasses.dataclass
+class Inputs:
+    """Hidden inputs of the hot path (the reference's parameters_module globals)."""
+    wl: Workload
+    hk: np.ndarray        # (n, n, nkp)
+    siw: np.ndarray       # (2*iwmax, n), row iw+iwmax holds nu_iw
+    giw: np.ndarray       # (2*iwmax, n)
+    dc: np.ndarray        # (2, n); only dc[0, :] enters G
+    mu: float
+    beta: float
+    u: np.ndarray         # (n^2, n^2)
+    u_tilde: np.ndarray   # (n^2, n^2)
+    n_dmft: np.ndarray    # (n,) complex
+    q_data: np.ndarray    # (nqp,) 1-based k indices of the q points
+    k_data_eom: np.ndarray  # (nkp_eom,) 1-based k indices of the EOM k points
+
+    @property
+    def n(self):
+        return self.wl.n
+
+    @property
+    def iwmax(self):
+        return self.siw.shape[0] // 2
+
+    @property
+    def iw_data(self):
+        i = np.arange(-self.iwmax, self.iwmax)
+        return math.pi * (2 * i + 1) / self.beta
+
+
+def _kgrid(nk):
+    nkx, nky, nkz = nk
+    ix, iy, iz = np.meshgrid(np.arange(nkx), np.arange(nky), np.arange(nkz), indexing="ij")
+    # kz fastest: flat index = kz + ky*nkz + kx*nky*nkz
+    kx = 2 * math.pi * ix.reshape(-1) / nkx
+    ky = 2 * math.pi * iy.reshape(-1) / nky
+    kz = 2 * math.pi * iz.reshape(-1) / nkz
+    return kx, ky, kz
+
+
+def _hk_sc1(wl):
+    kx, ky, kz = _kgrid(wl.nk)
+    e = -2.0 * wl.t * (np.cos(kx) + np.cos(ky) + np.cos(kz))
+    return e.reshape(1, 1, -1).astype(np.complex128)
+
+
+def _hk_t2g(wl):
+    kx, ky, kz = _kgrid(wl.nk)
+    t, tp = wl.t, wl.tp
+    nkp = kx.size
+    h = np.zeros((3, 3, nkp), dtype=np.complex128)
+    # orbitals: 0 = xy, 1 = yz, 2 = xz
+    h[0, 0] = 2 * t * (np.cos(kx) + np.cos(ky)) + 4 * tp * np.cos(kx) * np.cos(ky)
+    h[1, 1] = 2 * t * (np.cos(ky) + np.cos(kz)) + 4 * tp * np.cos(ky) * np.cos(kz)
+    h[2, 2] = 2 * t * (np.cos(kx) + np.cos(kz)) + 4 * tp * np.cos(kx) * np.cos(kz)
+    # inter-orbital NNN terms make G non-diagonal
+    h[1, 2] = h[2, 1] = 4 * tp * np.sin(kx) * np.sin(ky)
+    h[0, 2] = h[2, 0] = 4 * tp * np.sin(ky) * np.sin(kz)
+    h[0, 1] = h[1, 0] = 4 * tp * np.sin(kx) * np.sin(kz)
+    return h
+
+
+def _hk_rand(wl):
+    rng = np.random.default_rng([wl.seed, 7])
+    n = wl.n
+    kx, ky, kz = _kgrid(wl.nk)
+    shells = [(0, 0, 0),
+              (1, 0, 0), (0, 1, 0), (0, 0, 1),
+              (1, 1, 0), (1, -1, 0), (1, 0, 1), (1, 0, -1), (0, 1, 1), (0, 1, -1),
+              (1, 1, 1), (1, 1, -1), (1, -1, 1), (-1, 1, 1)]
+    h = np.zeros((n, n, kx.size), dtype=np.complex128)
+    for r in shells:
+        a = rng.normal(scale=wl.t, size=(n, n))
+        a = 0.5 * (a + a.T)
+        if r == (0, 0, 0):
+            h += a[:, :, None]
+        else:
+            ph = np.cos(r[0] * kx + r[1] * ky + r[2] * kz)
+            h += 2.0 * a[:, :, None] * ph[None, None, :]
+    return h
+
+
+def kanamori_u(n, U, J):
+    """U and U~ in compound index, following create_u (interaction_module.f90:182-291).
+
+    Umat(i,j,k,l): i=j=k=l -> U; i=j, k=l, i!=k -> J (pair hopping);
+    i=k, j=l, i!=j -> U-2J; i=l, j=k, i!=j -> J (spin flip).
+    u(i1={i,k}, i2={j,l}) = Umat(i,j,k,l), u~(i1, i2) = Umat(i,j,l,k) with
+    i1 = i*n + k and i2 = l*n + j (0-based).
+    """
+    V = U - 2.0 * J
+    umat = np.zeros((n, n, n, n))
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for l in range(n):
+                    if i == j and k == l:
+                        if i == k:
+                            umat[i, j, k, l] = U
+                            continue
+                        umat[i, j, k, l] = J
+                        continue
+                    if i == k and j == l:
+                        umat[i, j, k, l] = V
+                        continue
+                    if i == l and j == k:
+                        umat[i, j, k, l] = J
+    u = np.zeros((n * n, n * n), dtype=np.complex128)
+    ut = np.zeros_like(u)
+    for l in range(n):
+        for j in range(n):
+            i2 = l * n + j
+            for i in range(n):
+                for k in range(n):
+                    i1 = i * n + k
+                    u[i1, i2] = umat[i, j, k, l]
+                    ut[i1, i2] = umat[i, j, l, k]
+    return u, ut
+
+
+def k_index(kx, ky, kz, nk):
+    """1-based k index (kq_tools.f90:119-125)."""
+    return 1 + kz + ky * nk[2] + kx * nk[1] * nk[2]
+
+
+def generate_q_vol(nk, nq):
+    """q list on a sub-grid of the k grid (kq_tools.f90:42-57), 1-based."""
+    out = []
+    for i in range(nq[0]):
+        for j in range(nq[1]):
+            for k in range(nq[2]):
+                out.append(k_index(i * nk[0] // nq[0], j * nk[1] // nq[1], k * nk[2] // nq[2], nk))
+    return np.array(out, dtype=np.int32)
+
+
+def k_minus_q(ik, iq, nk):
+    """1-based index of k - q with fold-back by integer modulo (kq_tools.f90:96-108)."""
+    ik = np.asarray(ik) - 1
+    iq = np.asarray(iq) - 1
+    nkx, nky, nkz = nk
+    iz, iy, ix = ik % nkz, (ik // nkz) % nky, ik // (nky * nkz)
+    lz, ly, lx = iq % nkz, (iq // nkz) % nky, iq // (nky * nkz)
+    return (1 + (nkz + iz - lz) % nkz + ((nky + iy - ly) % nky) * nkz
+            + ((nkx + ix - lx) % nkx) * nky * nkz)
+
+
+def build_inputs(wl: Workload) -> Inputs:
+    n = wl.n
+    beta = wl.beta
+    iwmax = wl.n_iwmax
+    if iwmax < wl.iwfmax_small + wl.iwbmax_small:
+        raise ValueError("iwmax must be >= iwfmax_small + iwbmax_small (config_module.f90:600-607)")
+    if wl.model == "sc1":
+        hk = _hk_sc1(wl)
+    elif wl.model == "t2g":
+        hk = _hk_t2g(wl)
+    else:
+        hk = _hk_rand(wl)
+    assert hk.shape[0] == n
+    rng = np.random.default_rng([wl.seed, 1])
+    # Hubbard-I-like orbital-diagonal self-energy
+    if n == 1:
+        h = np.array([wl.U / 2])
+        s = np.array([wl.U / 2])
+        e = np.zeros(1)
+    else:
+        h = np.full(n, wl.U / 2) + 0.02 * wl.U * rng.normal(size=n)
+        s = wl.U / 2 * (1.0 + 0.05 * rng.normal(size=n))
+        e = 0.05 * wl.U * rng.normal(size=n)
+    mu = float(wl.U / 2)
+    iw = np.arange(-iwmax, iwmax)
+    nu = math.pi * (2 * iw + 1) / beta
+    siw = h[None, :] + s[None, :] ** 2 / (1j * nu[:, None] - e[None, :])
+    dc = np.zeros((2, n), dtype=np.complex128)
+    # g(nu) = 1/nkp sum_k diag G(k, nu) for nu >= 0, conjugate for nu < 0 (get_giw)
+    giw = np.zeros((2 * iwmax, n), dtype=np.complex128)
+    hkt = np.ascontiguousarray(hk.transpose(2, 0, 1))  # (nkp, n, n)
+    eye = np.eye(n)
+    for i0 in range(0, iwmax, 64):
+        ws = np.arange(i0, min(iwmax, i0 + 64))
+        a = -hkt[None, :, :, :] + np.zeros((ws.size, 1, 1, 1))
+        diag = (1j * nu[ws + iwmax][:, None] + mu - dc[0][None, :] - siw[ws + iwmax])  # (nw, n)
+        a = a + diag[:, None, :, None] * eye[None, None, :, :]
+        g = np.linalg.inv(a)
+        giw[ws + iwmax] = np.einsum("wkaa->wa", g) / hk.shape[2]
+    for w in range(iwmax):
+        giw[-w - 1 + iwmax] = np.conj(giw[w + iwmax])
+    # occupations (get_ndmft): n = 2 Re sum_{nu>=0} g / beta + 1/2
+    n_dmft = (2.0 * np.real(giw[iwmax:].sum(axis=0)) / beta + 0.5).astype(np.complex128)
+    u, ut = kanamori_u(n, wl.U, wl.J)
+    q_data = generate_q_vol(wl.nk, wl.qgrid)
+    k_eom = np.arange(1, wl.nkp + 1, dtype=np.int32)
+    return Inputs(wl, np.asfortranarray(hk), np.asfortranarray(siw), np.asfortranarray(giw),
+                  np.asfortranarray(dc), mu, beta, np.asfortranarray(u), np.asfortranarray(ut),
+                  n_dmft, q_data, k_eom)
+
+
+def chi_loc(inp: Inputs, iwb: int, channel: str) -> np.ndarray:
+    """Local generalized susceptibility chi_loc_r(omega) in the compound index that
+    `read_vertex` produces (hdf5_module.f90:708-770):
+    row i1 = nu1_idx*n^2 + i*n + j, column i2 = nu2_idx*n^2 + l*n + k (0-based).
+    Local bubble -beta g_i(nu) g_j(nu-omega) on the diagonal plus seeded Hermitian noise.
+    """
+    wl = inp.wl
+    n, nf = wl.n, wl.nf
+    N = n * n * nf
+    off = inp.iwmax
+    rch = {"dens": 0, "magn": 1}[channel]
+    rng = np.random.default_rng([wl.seed, 100 + rch, 1000 + iwb])
+    x = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    a = wl.noise * 0.5 * (x + x.conj().T)
+    diag = np.empty(N, dtype=np.complex128)
+    for d in range(nf):
+        iwf = d - wl.iwfmax_small
+        gi = inp.giw[iwf + off]
+        gj = inp.giw[iwf - iwb + off]
+        diag[d * n * n:(d + 1) * n * n] = (-inp.beta * gi[:, None] * gj[None, :]).reshape(-1)
+    a[np.arange(N), np.arange(N)] += diag
+    return np.asfortranarray(a)

@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput of the per-(q,omega) hot path on one MI355X.
+
+Workload: the 3-orbital t2g cubic model (NN t=-0.26, NNN t'=-0.08 eV, Kanamori U=5, J=0.75,
+beta=10), 20^3 k grid with q on the same grid, 2x60 fermionic x 31 bosonic frequencies, i.e.
+BSE matrices of N = 1080 (complex), seeded synthetic Sigma_loc and chi_loc.  A step is one pass
+of the hot path (bubble, BSE build, density + magnetic solves, eta, chi, EOM) over one batch of
+`--batch` q points at one bosonic frequency; the local update (set_local_omega) runs inside the
+timed region whenever the frequency changes.  `value` is refined-mode (complex64 LU on the GPU,
+FP64 residuals and iterative refinement) points/s; the plain FP64 path is timed on the same
+points and reported beside it, with the per-quantity difference between the two.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "(q,ω) BSE+EOM points/sec, refined-FP32 vs plain FP64, at 1e-10 rel err"
+PEAK = {"refined": 157.3, "fp64": 78.6}  # MI355X dense MFMA TFLOP/s: FP32 and FP64
+
+
+def lu_flops(N, n2):
+    """Algorithmic flops of one BSE solve: complex LU (8/3 N^3) + n2 right-hand sides (8 N^2 n2)."""
+    return 8.0 / 3.0 * N ** 3 + 8.0 * N * N * n2
+
+
+def read_traffic(name):
+    """HBM bytes per LU launch from the committed rocprofv3 --pmc summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache):
+    wl = inp.wl
+    nqp = wl.nqp
+    blocks_per_omega = math.ceil(nqp / batch)
+    ctx = api.Context(inp, batch=batch)
+    cur = [None]
+
+    def step(s):
+        iwb = omegas[(s // blocks_per_omega) % len(omegas)]
+        blk = s % blocks_per_omega
+        if iwb != cur[0]:
+            ctx.set_local_omega(iwb, *chi_cache(iwb))
+            cur[0] = iwb
+        q0 = blk * batch
+        iq = np.arange(q0, min(nqp, q0 + batch), dtype=np.int32) + 1
+        return ctx.run_points(iq, precision)
+
+    for s in range(warmup):
+        step(s)
+    ctx.reset_sigma()
+    cur[0] = None
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    lu_ms, lu_n, iters, resid = 0.0, 0, 0.0, 0.0
+    outs = []
+    npts = 0
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        r = step(s)
+        st = ctx.stats()
+        lu_ms += st["t_lu_kernel"]
+        lu_n += int(st["n_lu_launches"])
+        iters = max(iters, st["refine_iters"])
+        resid = max(resid, st["refine_resid"])
+        npts += r["chi_d"].shape[2]
+        outs.append(r)
+    t1 = time.perf_counter()
+    sigma = ctx.get_sigma()[0]
+    ctx.close()
+    return {"seconds": t1 - t0, "points": npts, "lu_ms": lu_ms, "lu_launches": lu_n, "iters": iters,
+            "resid": resid, "chi_d": np.concatenate([o["chi_d"] for o in outs], axis=2),
+            "chi_m": np.concatenate([o["chi_m"] for o in outs], axis=2), "sigma_nl": sigma}
+
+
+def cpu_baseline(synth, inp, chi_cache, npoints):
+    """The reference's own Fortran modules (oracle/_ref) on one host core, or the numpy oracle when the
+    reference binary is absent; per-point time from the difference of a 1-point and a (1+npoints)-point run
+    (removes the harness's set-up)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_runner
+    cl = lambda w, c: chi_cache(w)[0 if c == "dens" else 1]  # noqa: E731
+    if ref_runner.available():
+        t1 = ref_runner.run(inp, [(0, 0)], cl, threads=1)["wall_s"]
+        tn = ref_runner.run(inp, [(0, q) for q in range(npoints + 1)], cl, threads=1)["wall_s"]
+        kind = "reference"
+        desc = "oracle/_ref (reference src/*_module.f90 via amdflang + MKL, 1 thread)"
+    else:
+        import adga_oracle
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(1):
+            hp = adga_oracle.HotPath(inp)
+            a = time.perf_counter()
+            hp.run([(0, 0)], cl)
+            t1 = time.perf_counter() - a
+            a = time.perf_counter()
+            hp.run([(0, q) for q in range(npoints + 1)], cl)
+            tn = time.perf_counter() - a
+        kind = "port"
+        desc = "numpy oracle, 1 thread"
+    per = max((tn - t1) / npoints, 1e-9)
+    return {"value": 1.0 / per, "unit": "points/s", "cores": 1, "kind": kind,
+            "sample": f"{npoints} (q,omega) points at omega=0 of the same workload, {desc}; "
+                      f"{per:.3f} s/point"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--workload", default="t2g_k20")
+    ap.add_argument("--cpu-points", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the hot path is built and measured on a single MI355X; "
+              "sharding across GPUs is a design note only (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from adga_amd import api, synth
+    wl = synth.workload(args.workload)
+    inp = synth.build_inputs(wl)
+    omegas = [0, 1, -1, 2, -2, 3, -3]
+    cache = {}
+
+    def chi_cache(iwb):
+        if iwb not in cache:
+            cache[iwb] = (synth.chi_loc(inp, iwb, "dens"), synth.chi_loc(inp, iwb, "magn"))
+        return cache[iwb]
+
+    nsteps = args.steps + args.warmup
+    blocks = math.ceil(wl.nqp / args.batch)
+    for s in range(nsteps):
+        chi_cache(omegas[(s // blocks) % len(omegas)])
+
+    ref = run_mode(api, synth, inp, "refined", args.batch, args.steps, args.warmup, omegas, chi_cache)
+    fp = None if args.no_fp64 else run_mode(api, synth, inp, "fp64", args.batch, args.steps, args.warmup,
+                                             omegas, chi_cache)
+    value = ref["points"] / ref["seconds"]
+    N, n2 = wl.maxdim, wl.n * wl.n
+    nmat_per_launch = 2 * args.batch
+    flops_launch = nmat_per_launch * lu_flops(N, n2)
+    avg_lu_s = ref["lu_ms"] / max(ref["lu_launches"], 1) / 1e3
+    achieved = flops_launch / avg_lu_s / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * ref["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 LU + f64 refinement",
+        "data": "synthetic (seeded t2g tight-binding H(k), Hubbard-I Sigma, chi_loc = local bubble + 1e-3 noise)",
+        "config": {"workload": "t2g_k20: 3-orbital t2g cubic, 20^3 k, q = k grid, 2x60 fermionic x 31 bosonic, "
+                               "BSE N=1080 complex, density + magnetic channels",
+                   "batch_points": args.batch, "precision": "refined", "refine_tol": 1e-13},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["refined"], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK["refined"], 4),
+                     "traffic": read_traffic("k_lu_float_t2g_k20_traffic.md"),
+                     "kernel": "k_lu<float> (batched complex LU + n^2 solves, one workgroup per matrix)",
+                     "flops_per_launch": flops_launch, "avg_launch_ms": round(avg_lu_s * 1e3, 3)},
+        "refine_iters_max": ref["iters"],
+        "refine_resid_max": ref["resid"],
+    }
+    if fp is not None:
+        fval = fp["points"] / fp["seconds"]
+        favg = fp["lu_ms"] / max(fp["lu_launches"], 1) / 1e3
+        errs = {}
+        for k in ("chi_d", "chi_m", "sigma_nl"):
+            den = np.abs(fp[k]).max()
+            errs[k] = float(np.abs(ref[k] - fp[k]).max() / den) if den > 0 else 0.0
+        line["plain_fp64"] = {"value": round(fval, 3), "unit": "points/s",
+                              "ms_per_step": round(1e3 * fp["seconds"] / args.steps, 3),
+                              "lu_achieved_tflops": round(flops_launch / favg / 1e12, 3),
+                              "lu_frac_of_fp64_peak": round(flops_launch / favg / 1e12 / PEAK["fp64"], 4)}
+        line["refined_over_fp64"] = round(value / fval, 3)
+        line["rel_err_refined_vs_fp64"] = errs
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(synth, inp, chi_cache, args.cpu_points)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

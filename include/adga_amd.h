@@ -1,0 +1,134 @@
+/*
+ * adga_amd.h - C ABI of the MI355X (gfx950) implementation of ADGA's per-(q,omega) hot path:
+ * the chi0 bubble, the density/magnetic Bethe-Salpeter solves, the chi(q,omega) contraction and
+ * the equation of motion for Sigma_DGA(k,nu).
+ *
+ * Conventions (all entry points):
+ *   - complex arrays are interleaved (re, im) doubles, i.e. Fortran complex(kind=8), passed as
+ *     `double *` and laid out column-major with the reference's Fortran shapes (first index
+ *     fastest); frequency axes keep the reference's offsets (siw(-iwmax:iwmax-1, ndim) is passed
+ *     as a (2*iwmax, ndim) array whose row iw+iwmax holds nu_iw);
+ *   - integers are int32; q and k indices are the reference's 1-based k-grid indices
+ *     (ik = 1 + kz + ky*nkz + kx*nky*nkz, src/kq_tools.f90:119-125);
+ *   - every entry point returns an ADGA_* status code and never throws across the boundary;
+ *     adga_last_error() gives the message of the last failure on a context.
+ * Arrays are caller-owned host memory; the library copies what it needs to the GPU.
+ */
+#ifndef ADGA_AMD_H
+#define ADGA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ADGA_OK = 0,
+  ADGA_E_ARG = 1,        /* invalid argument or size */
+  ADGA_E_HIP = 2,        /* HIP runtime failure (no device, launch failure, ...) */
+  ADGA_E_SINGULAR = 3,   /* exactly zero pivot in a BSE LU: the reference's "ERROR in ZGETRF" */
+  ADGA_E_NOCONV = 4,     /* refined mode: residual above tolerance after the iteration limit */
+  ADGA_E_NONFINITE = 5,  /* NaN/Inf met in a BSE matrix or residual */
+  ADGA_E_STATE = 6,      /* call out of order (e.g. run_points before set_local_omega) */
+  ADGA_E_NOMEM = 7       /* device allocation failed */
+};
+
+enum {
+  ADGA_PREC_FP64 = 0,    /* complex128 LU and solve */
+  ADGA_PREC_REFINED = 1  /* complex64 LU, FP64 residuals and iterative refinement */
+};
+
+typedef struct adga_ctx adga_ctx;
+
+/* Sizes and scalars of the reference's parameters_module (src/parameters_module.f90:43-73). */
+typedef struct adga_dims {
+  int ndim;                 /* orbitals n */
+  int nkpx, nkpy, nkpz;     /* k grid (nkp = product) */
+  int iwmax;                /* 1P frequencies: siw/giw hold 2*iwmax entries */
+  int iwfmax_small;         /* fermionic box: Nf = 2*iwfmax_small, BSE size N = n^2*Nf */
+  int iwbmax_small;         /* bosonic box: omega in [-iwbmax_small, iwbmax_small] */
+  int nqp;                  /* length of the q list */
+  int nkp_eom;              /* length of the EOM k list */
+  double beta, mu;
+} adga_dims;
+
+/*
+ * Creates a context on the current HIP device and uploads the one-particle data (the hidden
+ * inputs of the hot routines: hk, siw, giw, dc, u, u_tilde, n_dmft, q_data, k_data_eom), then
+ * builds the device table G(k, nu) for every nu the loop needs.
+ *   hk(ndim,ndim,nkp), siw/giw(2*iwmax,ndim), dc(2,ndim), u/u_tilde(ndim^2,ndim^2), n_dmft(ndim),
+ *   q_data(nqp), k_data_eom(nkp_eom).
+ * Replaces: the setup the reference does before its loop (src/main.F90:174-389) for the data the
+ * hot routines read through parameters_module.
+ */
+int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_eom, const double *hk,
+                    const double *siw, const double *giw, const double *dc, const double *u,
+                    const double *u_tilde, const double *n_dmft, adga_ctx **ctx);
+
+/*
+ * Options: "batch" (points per device batch), "refine_tol" (relative residual max|r|/max|b|),
+ * "refine_maxit".  Returns ADGA_E_ARG for an unknown key or value.
+ */
+int adga_set_option(adga_ctx *ctx, const char *key, double value);
+
+/*
+ * Local update for bosonic frequency iwb: chi0^w and its inverse, chi0^w F_r = chi_loc_r chi0^w^-1 - 1
+ * after the transpose symmetrisation, gamma^w_r and 1 + gamma^w_r.
+ *   chi_loc_dens/magn(N,N): what read_vertex returns (src/hdf5_module.f90:708-770).
+ *   Outputs (may be NULL): chi_loc_dens_out/magn_out(ndim^2,ndim^2) = calc_chi_qw(gamma^w, chi0^w)
+ *   and bubble_loc_out(ndim^2,ndim^2) = sum_nu chi0^w / beta^2, which the reference accumulates when
+ *   iq == 1.
+ * Replaces: src/main.F90:506-625 (get_chi0_loc_inv, get_chi0_loc, symmetrisation, chi0^w F, gamma^w).
+ */
+int adga_set_local_omega(adga_ctx *ctx, int iwb, const double *chi_loc_dens, const double *chi_loc_magn,
+                         double *chi_loc_dens_out, double *chi_loc_magn_out, double *bubble_loc_out);
+
+/*
+ * Runs the per-point work for npts q points at the current bosonic frequency: bubble, BSE build,
+ * both channel solves, eta, the chi contraction and the dynamic EOM (Sigma_nl accumulated on the
+ * device), plus the local EOM for every point with iq == 1 (calc_eom_static at iwb == 0 and
+ * calc_eom_dmft).
+ *   iq(npts): 1-based positions in q_data.
+ *   chi_dens_out/chi_magn_out/bubble_nl_out(ndim^2,ndim^2,npts): the non-local chi parts and the
+ *   non-local bubble, as chi_qw_dens/magn(:,:,iqw) and bubble_nl(:,:,iqw) of the reference.
+ *   status(npts): per-point ADGA_* code (may be NULL).
+ * Replaces: the loop body src/main.F90:635-781 (accumulate_chi0, the inline BSE zgemm calls,
+ * inverse_matrix, calc_chi_qw, calc_eom_dynamic/_dmft/_static) for a batch of points.
+ */
+int adga_run_points(adga_ctx *ctx, int npts, const int *iq, int precision, double *chi_dens_out,
+                    double *chi_magn_out, double *bubble_nl_out, int *status);
+
+/*
+ * Downloads the accumulated self-energies, with Sigma(-nu-1) = Sigma(nu)^H filled in:
+ *   sigma_nl(ndim,ndim,2*iwfmax_small,nkp_eom), sigma_dmft(ndim,ndim,2*iwfmax_small),
+ *   sigma_hf(ndim,ndim,nkp_eom) (zero: no V(q)).  Any pointer may be NULL.
+ * Replaces: reading sigma_nl/sigma_dmft/sigma_hf after the loop (src/main.F90:841-855).
+ */
+int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *sigma_hf);
+
+/* Zeroes the accumulated self-energies. */
+int adga_reset_sigma(adga_ctx *ctx);
+
+/*
+ * Statistics of the last adga_run_points call: out[0] = max refinement iterations,
+ * out[1] = max final relative residual, out[2..8] = milliseconds spent in the reference's stage
+ * buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM, Chi (src/main.F90:484-783).
+ */
+int adga_get_stats(adga_ctx *ctx, double *out, int nout);
+
+/* Copies the message of the last failure (NUL-terminated, truncated to len). */
+int adga_last_error(adga_ctx *ctx, char *buf, int len);
+
+int adga_ctx_destroy(adga_ctx *ctx);
+
+/*
+ * In-place inverse of a complex(kind=8) n x n column-major matrix on the GPU (LU with partial
+ * pivoting, then n solves); ierr > 0 is the first zero pivot like zgetrf's info.
+ * Replaces: inverse_matrix_z (src/lapack_module.f90:40-70).
+ */
+int adga_inverse_matrix_z(int n, double *m, int *ierr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADGA_AMD_H */

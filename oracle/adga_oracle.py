@@ -1,0 +1,328 @@
+"""CPU oracle (numpy, complex128) of ADGA's per-(q,omega) hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: only
+`tests/`, `__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py`
+may import it, and never as the thing that is measured or shipped.
+
+It restates the reference's algorithm in numpy, keeping the reference's array
+shapes and index meaning (first index fastest, 0-based here):
+
+* G(k,nu)                   one_particle_quant_module.f90:81-104, 240-271; aux.f90:288-325
+* k - q                     kq_tools.f90:96-108
+* chi0^q bubble             one_particle_quant_module.f90:228-290; main.F90:644-657
+* chi0^w and its inverse    one_particle_quant_module.f90:192-225; main.F90:509-522
+* local vertex preparation  main.F90:525-625 (chi_loc from read_vertex, hdf5_module.f90:708-770)
+* BSE build                 main.F90:673-720
+* BSE inversion             main.F90:726-743 -> lapack_module.f90:40-70 (zgetrf + zgetri)
+* eta                       main.F90:749-757
+* chi contraction           susc_module.f90:34-56; main.F90:573-577, 661-668, 766-773
+* EOM (dynamic)             eom_module.f90:165-233
+* EOM (local, static)       eom_module.f90:39-161; main.F90:635-639
+* Sigma assembly            eom_module.f90:237-251; main.F90:840-855
+* chi assembly              main.F90:930-936, 1047-1063, 1124-1140
+
+Parity pins (tests/test_oracle_*.py): golden vectors written by the reference's
+own Fortran modules (oracle/_ref, built from the reference sources with
+amdflang + MKL) on small workloads, and closed forms: the F = 0 limit, the
+local limit, a brute-force loop restatement on tiny grids, numpy inverse for the
+adjugate, and the Matsubara symmetries.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = ["HotPath", "inverse_small", "kmq_table"]
+
+
+def inverse_small(a):
+    """Batched inverse of (..., n, n) matrices: closed-form adjugate for n <= 3
+    (aux.f90:288-325, main routine's choice at one_particle_quant_module.f90:245-256),
+    LAPACK otherwise (lapack_module.f90:40-70)."""
+    n = a.shape[-1]
+    if n == 1:
+        return 1.0 / a
+    if n == 2:
+        d = 1.0 / (a[..., 0, 0] * a[..., 1, 1] - a[..., 0, 1] * a[..., 1, 0])
+        b = np.empty_like(a)
+        b[..., 0, 0] = d * a[..., 1, 1]
+        b[..., 1, 0] = -d * a[..., 1, 0]
+        b[..., 0, 1] = -d * a[..., 0, 1]
+        b[..., 1, 1] = d * a[..., 0, 0]
+        return b
+    if n == 3:
+        A = lambda i, j: a[..., i - 1, j - 1]  # noqa: E731  (1-based like the formula)
+        d = 1.0 / (A(1, 1) * A(2, 2) * A(3, 3) - A(1, 1) * A(2, 3) * A(3, 2)
+                   - A(1, 2) * A(2, 1) * A(3, 3) + A(1, 2) * A(2, 3) * A(3, 1)
+                   + A(1, 3) * A(2, 1) * A(3, 2) - A(1, 3) * A(2, 2) * A(3, 1))
+        b = np.empty_like(a)
+        b[..., 0, 0] = d * (A(2, 2) * A(3, 3) - A(2, 3) * A(3, 2))
+        b[..., 1, 0] = -d * (A(2, 1) * A(3, 3) - A(2, 3) * A(3, 1))
+        b[..., 2, 0] = d * (A(2, 1) * A(3, 2) - A(2, 2) * A(3, 1))
+        b[..., 0, 1] = -d * (A(1, 2) * A(3, 3) - A(1, 3) * A(3, 2))
+        b[..., 1, 1] = d * (A(1, 1) * A(3, 3) - A(1, 3) * A(3, 1))
+        b[..., 2, 1] = -d * (A(1, 1) * A(3, 2) - A(1, 2) * A(3, 1))
+        b[..., 0, 2] = d * (A(1, 2) * A(2, 3) - A(1, 3) * A(2, 2))
+        b[..., 1, 2] = -d * (A(1, 1) * A(2, 3) - A(1, 3) * A(2, 1))
+        b[..., 2, 2] = d * (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1))
+        return b
+    return np.linalg.inv(a)
+
+
+def kmq_table(nk, q_data):
+    """kq_ind(nkp, nqp): 0-based index of k - q for every k and every q of the list
+    (index_kq, kq_tools.f90:59-71 with k_minus_q 96-108)."""
+    nkx, nky, nkz = nk
+    ik = np.arange(nkx * nky * nkz)
+    iz, iy, ix = ik % nkz, (ik // nkz) % nky, ik // (nky * nkz)
+    q = np.asarray(q_data) - 1
+    lz, ly, lx = q % nkz, (q // nkz) % nky, q // (nky * nkz)
+    return (((iz[:, None] - lz[None, :]) % nkz) + ((iy[:, None] - ly[None, :]) % nky) * nkz
+            + ((ix[:, None] - lx[None, :]) % nkx) * nky * nkz)
+
+
+class HotPath:
+    """The per-(q,omega) loop body of main.F90:486-799 over given inputs."""
+
+    def __init__(self, inp, bse_inverse=None):
+        self.inp = inp
+        wl = inp.wl
+        self.n = wl.n
+        self.n2 = wl.n * wl.n
+        self.iwf = wl.iwfmax_small
+        self.iwb = wl.iwbmax_small
+        self.nf = 2 * wl.iwfmax_small
+        self.N = self.n2 * self.nf
+        self.nkp = inp.hk.shape[2]
+        self.nqp = len(inp.q_data)
+        self.nkp_eom = len(inp.k_data_eom)
+        self.beta = inp.beta
+        self.off = inp.iwmax
+        self.kq = kmq_table(wl.nk, inp.q_data)
+        # frequencies used by G(k, nu) and G(k-q, nu-omega)
+        self.glo = -self.iwf - self.iwb
+        self.ghi = self.iwf + self.iwb - 1
+        self._gtab = None
+        self.bse_inverse = bse_inverse or np.linalg.inv
+
+    # ---------------------------------------------------------------- A1: G(k, nu)
+    def g_matrix(self, iks, iw):
+        """G(k, nu_iw) for 0-based k indices `iks`, shape (len, n, n) with [.., a, b]
+        = G_ab (one_particle_quant_module.f90:240-271, get_gkiw 81-104)."""
+        inp = self.inp
+        n = self.n
+        hk = inp.hk[:, :, iks]  # (n, n, K)
+        a = -np.transpose(hk, (2, 0, 1)).copy()
+        nu = inp.iw_data[iw + self.off]
+        for i in range(n):
+            a[:, i, i] = 1j * nu + inp.mu - hk[i, i, :] - inp.dc[0, i] - inp.siw[iw + self.off, i]
+        return inverse_small(a)
+
+    @property
+    def gtab(self):
+        """G table (n_w, nkp, n, n) over nu in [glo, ghi]."""
+        if self._gtab is None:
+            allk = np.arange(self.nkp)
+            self._gtab = np.stack([self.g_matrix(allk, w) for w in range(self.glo, self.ghi + 1)])
+        return self._gtab
+
+    def G(self, iw):
+        return self.gtab[iw - self.glo]
+
+    # ---------------------------------------------------------------- A4
+    def chi0_loc(self, iwf, iwb):
+        """Diagonal of chi0^w(nu) in compound index i1 = i*n + j (get_chi0_loc 192-207)."""
+        g = self.inp.giw
+        gi = g[iwf + self.off]
+        gj = g[iwf - iwb + self.off]
+        return (-self.beta * gi[:, None] * gj[None, :]).reshape(-1)
+
+    def chi0_loc_inv(self, iwf, iwb):
+        """Diagonal of [chi0^w(nu)]^-1 (get_chi0_loc_inv 210-225)."""
+        g = self.inp.giw
+        gi = g[iwf + self.off]
+        gj = g[iwf - iwb + self.off]
+        return (-1.0 / (self.beta * gi[:, None] * gj[None, :])).reshape(-1)
+
+    # ---------------------------------------------------------------- A3
+    def chi0q(self, iq, iwb):
+        """k-summed bubble chi0^q(nu) for every nu of the box, shape (n2, n2, Nf):
+        chi0[(i,j),(l,k)] = -beta/nkp sum_k G(k,nu)_il G(k-q,nu-omega)_kj
+        (accumulate_chi0 one_particle_quant_module.f90:274-288, main.F90:647-657)."""
+        n = self.n
+        kq = self.kq[:, iq]
+        out = np.empty((self.n2, self.n2, self.nf), dtype=np.complex128)
+        for d in range(self.nf):
+            iwf = d - self.iwf
+            g1 = self.G(iwf)            # (K, n, n) [K, i, l]
+            g2 = self.G(iwf - iwb)[kq]  # (K, n, n) [K, k, j]
+            c = -self.beta * np.einsum("Kil,Kkj->ijlk", g1, g2, optimize=True)
+            out[:, :, d] = c.reshape(self.n2, self.n2) / self.nkp
+        return out
+
+    # ---------------------------------------------------------------- A9
+    def calc_chi_qw(self, x, y):
+        """beta^-2 sum_{i3,nu} X(:,{i3,nu}) Y(i3,:,nu) (susc_module.f90:34-56)."""
+        yy = np.transpose(y, (2, 0, 1)).reshape(self.N, self.n2)
+        return (x @ yy) / self.beta ** 2
+
+    # ---------------------------------------------------------------- A4 + A5
+    def local_omega(self, iwb, chi_loc_d, chi_loc_m):
+        """Quantities updated when omega changes (main.F90:506-625)."""
+        n2, nf, N = self.n2, self.nf, self.N
+        winv = np.stack([self.chi0_loc_inv(d - self.iwf, iwb) for d in range(nf)], axis=1)  # (n2, nf)
+        w = np.stack([self.chi0_loc(d - self.iwf, iwb) for d in range(nf)], axis=1)
+        chi0w = np.zeros((n2, n2, nf), dtype=np.complex128)
+        for d in range(nf):
+            chi0w[np.arange(n2), np.arange(n2), d] = w[:, d]
+        bubble_loc_tmp = chi0w.sum(axis=2) / self.beta ** 2
+        out = {"winv": winv, "chi0w": chi0w, "bubble_loc_tmp": bubble_loc_tmp}
+        for r, cl in (("d", chi_loc_d), ("m", chi_loc_m)):
+            a = 0.5 * (cl + cl.T)                        # main.F90:529-537
+            a = a * winv.T.reshape(-1)[None, :]          # column scaling, main.F90:542-553
+            a[np.arange(N), np.arange(N)] -= 1.0
+            gam = a.reshape(nf, n2, N).sum(axis=0)       # main.F90:557-571
+            one = gam.copy()                             # main.F90:616-625
+            for d in range(nf):
+                one[np.arange(n2), d * n2 + np.arange(n2)] += 1.0
+            out["chi0wF_" + r] = a
+            out["gammaw_" + r] = gam
+            out["oneplusgammaw_" + r] = one
+            out["chi_loc_" + r] = self.calc_chi_qw(gam, chi0w)  # main.F90:573-577
+        return out
+
+    # ---------------------------------------------------------------- A6 - A9 for one point
+    def point(self, iq, iwb, loc, chi0q=None):
+        n2, nf, N = self.n2, self.nf, self.N
+        if chi0q is None:
+            chi0q = self.chi0q(iq, iwb)
+        res = {"chi0q": chi0q}
+        bubble_nl = chi0q.sum(axis=2) / self.beta ** 2 - loc["bubble_loc_tmp"]   # main.F90:663
+        chi0nl = chi0q - loc["chi0w"]
+        res["bubble_nl"] = bubble_nl
+        chi = {r: self.calc_chi_qw(loc["gammaw_" + r], chi0nl) for r in ("d", "m")}  # main.F90:666-667
+        big = {r: np.zeros((N, N), dtype=np.complex128) for r in ("d", "m")}
+        gammaqd = np.zeros((n2, N), dtype=np.complex128)
+        for d in range(nf):                                   # main.F90:679-720
+            s = chi0q[:, :, d] * loc["winv"][:, d][None, :]
+            s[np.arange(n2), np.arange(n2)] -= 1.0
+            rows = slice(d * n2, (d + 1) * n2)
+            for r in ("d", "m"):
+                rect = s @ loc["chi0wF_" + r][rows, :]
+                big[r][rows, :] = -rect
+                if r == "d":
+                    gammaqd += rect
+        res["gammaqd"] = gammaqd
+        for r in ("d", "m"):
+            a = big[r] + np.eye(N)                            # main.F90:726-729
+            res["A_" + r] = a
+            ainv = self.bse_inverse(a) - np.eye(N)           # main.F90:734-743
+            eta = loc["oneplusgammaw_" + r] @ ainv            # main.F90:749-757
+            res["eta_" + r] = eta
+            chi[r] = chi[r] + self.calc_chi_qw(eta, chi0q)    # main.F90:766-769
+            res["chi_" + r] = chi[r]
+        return res
+
+    # ---------------------------------------------------------------- A10
+    def mtot(self, etaqd, etaqm, gammaqd):
+        """m_tot of calc_eom_dynamic (eom_module.f90:187-201), no V(q)."""
+        inp = self.inp
+        f = 1.0 / (self.beta * self.nqp)
+        uw = inp.u - 0.5 * inp.u_tilde
+        return (-f) * (uw @ etaqd) + 1.5 * f * (inp.u_tilde @ etaqm) + f * (inp.u @ gammaqd)
+
+    def eom_dynamic(self, etaqd, etaqm, gammaqd, iq, iwb, sigma_nl):
+        """Sigma_nl(i,l,nu,k) += sum_{j,k'} m_tot({ji},{k'l,nu}) G(k-q,nu-omega)_{jk'}
+        for nu >= 0, then Sigma(-nu-1) = Sigma(nu)^H (eom_module.f90:211-231)."""
+        n, n2 = self.n, self.n2
+        m = self.mtot(etaqd, etaqm, gammaqd)
+        kidx = self.inp.k_data_eom - 1
+        for d in range(self.iwf):                 # iwf = 0 .. iwfmax_small-1
+            iwf = d
+            col0 = n2 * (self.iwf + iwf)
+            mm = m[:, col0:col0 + n2].reshape(n, n, n, n)  # [i, j, l, k']
+            g = self.G(iwf - iwb)[self.kq[kidx, iq]]        # (K, j, k')
+            add = np.einsum("ijlk,Kjk->ilK", mm, g, optimize=True)
+            sigma_nl[:, :, self.iwf + iwf, :] += add
+            sigma_nl[:, :, self.iwf - iwf - 1, :] = np.conj(np.transpose(sigma_nl[:, :, self.iwf + iwf, :], (1, 0, 2)))
+        return m
+
+    # ---------------------------------------------------------------- A11
+    def eom_dmft(self, gammawd, iwb, sigma_dmft):
+        """Local EOM with the DMFT G (eom_module.f90:115-161, default sig_ph branch)."""
+        n, n2 = self.n, self.n2
+        m = (-1.0 / self.beta) * (self.inp.u @ gammawd)
+        for iwf in range(self.iwf):
+            col0 = n2 * (self.iwf + iwf)
+            # m_tot(i1 = i*n + k, i2 = col0 + l*n + k): j == k because g is diagonal
+            mm = m[:, col0:col0 + n2].reshape(n, n, n, n)  # [i, k, l, k']
+            gl = self.inp.giw[iwf - iwb + self.off]
+            sigma_dmft[:, :, self.iwf + iwf] += np.einsum("iklk,k->il", mm, gl)
+            sigma_dmft[:, :, self.iwf - iwf - 1] = np.conj(sigma_dmft[:, :, self.iwf + iwf].T)
+        return m
+
+    def eom_static_local(self, sigma_dmft):
+        """Local Hartree term sum_j (2 u({ii},{jj}) - u({ji},{ji})) n_j (eom_module.f90:58-70)."""
+        n = self.n
+        u = self.inp.u
+        for i in range(n):
+            acc = 0.0
+            for j in range(n):
+                acc = acc + (2.0 * u[i * n + i, j * n + j] - u[i * n + j, i * n + j]) * self.inp.n_dmft[j]
+            sigma_dmft[i, i, :] += acc
+
+    # ---------------------------------------------------------------- the loop
+    def run(self, points, chi_loc_fn, keep=()):
+        """Run the loop body for `points` = [(iwb, iq0), ...] in the given order
+        (iq0 0-based into q_data).  `chi_loc_fn(iwb, channel)` supplies what
+        read_vertex would return.  Returns per-point and accumulated outputs."""
+        n, n2, nf = self.n, self.n2, self.nf
+        npts = len(points)
+        out = {
+            "chi_d": np.zeros((n2, n2, npts), dtype=np.complex128),
+            "chi_m": np.zeros((n2, n2, npts), dtype=np.complex128),
+            "bubble_nl": np.zeros((n2, n2, npts), dtype=np.complex128),
+            "sigma_nl": np.zeros((n, n, nf, self.nkp_eom), dtype=np.complex128),
+            "sigma_dmft": np.zeros((n, n, nf), dtype=np.complex128),
+            "sigma_hf": np.zeros((n, n, self.nkp_eom), dtype=np.complex128),
+            "chi_loc_d": {}, "chi_loc_m": {}, "bubble_loc": {},
+            "points": {},
+        }
+        cur = None
+        loc = None
+        for p, (iwb, iq) in enumerate(points):
+            if iwb != cur:
+                cur = iwb
+                loc = self.local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"))
+                if iq == 0:
+                    out["bubble_loc"][iwb] = loc["bubble_loc_tmp"]
+                    out["chi_loc_d"][iwb] = loc["chi_loc_d"]
+                    out["chi_loc_m"][iwb] = loc["chi_loc_m"]
+            if iwb == 0 and iq == 0:
+                self.eom_static_local(out["sigma_dmft"])
+            if iq == 0:
+                self.eom_dmft(loc["gammaw_d"], iwb, out["sigma_dmft"])
+            res = self.point(iq, iwb, loc)
+            out["chi_d"][:, :, p] = res["chi_d"]
+            out["chi_m"][:, :, p] = res["chi_m"]
+            out["bubble_nl"][:, :, p] = res["bubble_nl"]
+            self.eom_dynamic(res["eta_d"], res["eta_m"], res["gammaqd"], iq, iwb, out["sigma_nl"])
+            if (iwb, iq) in keep:
+                out["points"][(iwb, iq)] = res
+        return out
+
+    def finalize(self, out, points):
+        """chi_r(q,omega) = chi_nl + bubble_nl + chi_loc(omega) with chi_loc = calc_chi_qw(gamma, chi0w)
+        + bubble_loc (main.F90:935-936, 1047-1063, 1124-1140); Sigma_DGA = Sigma_nl + Sigma_DMFT
+        (add_siw_dmft, eom_module.f90:237-251)."""
+        fin = {}
+        for r in ("d", "m"):
+            tot = out["chi_" + r] + out["bubble_nl"]
+            for p, (iwb, iq) in enumerate(points):
+                if iwb in out["chi_loc_" + r]:
+                    tot[:, :, p] += out["chi_loc_" + r][iwb] + out["bubble_loc"][iwb]
+            fin["chi_" + r] = tot
+        s = out["sigma_nl"].copy()
+        for a in range(self.n):
+            s[a, a, :, :] += self.inp.siw[self.off - self.iwf:self.off + self.iwf, a][:, None]
+        fin["sigma_dga"] = s
+        return fin

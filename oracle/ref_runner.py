@@ -1,0 +1,75 @@
+"""Drives oracle/_ref/adga_ref_harness (the reference's own Fortran hot-path
+modules plus a test-only loop driver) on a synthetic workload.
+
+TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py to write golden
+vectors, by tests that compare the numpy oracle with the reference where the
+binary exists, and by bench.py's `cpu_baseline` leg.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(HERE, "_ref", "adga_ref_harness")
+
+
+def available() -> bool:
+    return os.path.exists(HARNESS)
+
+
+def _w(f, a, dtype):
+    np.asarray(a, dtype=dtype).flatten(order="F").tofile(f)
+
+
+def run(inp, points, chi_loc_fn, threads=1, timeout=3600):
+    """Run the reference modules over `points` = [(iwb, iq0), ...] (iq0 0-based).
+    Returns a dict of Fortran-shaped complex128 arrays and the wall time."""
+    wl = inp.wl
+    n, n2, nf = wl.n, wl.n * wl.n, wl.nf
+    N = n2 * nf
+    nw = 2 * wl.iwbmax_small + 1
+    npts = len(points)
+    nkp_eom = len(inp.k_data_eom)
+    wused = sorted({p[0] for p in points})
+    with tempfile.TemporaryDirectory() as td:
+        fi = os.path.join(td, "in.bin")
+        fo = os.path.join(td, "out.bin")
+        with open(fi, "wb") as f:
+            _w(f, [n, *wl.nk, inp.iwmax, wl.iwfmax_small, wl.iwbmax_small, len(inp.q_data),
+                   nkp_eom, npts, 0], np.int32)
+            _w(f, [inp.beta, inp.mu], np.float64)
+            for a in (inp.hk, inp.siw, inp.giw, inp.dc, inp.u, inp.u_tilde, inp.n_dmft):
+                _w(f, a, np.complex128)
+            _w(f, inp.q_data, np.int32)
+            _w(f, inp.k_data_eom, np.int32)
+            _w(f, [p[0] for p in points], np.int32)
+            _w(f, [p[1] + 1 for p in points], np.int32)
+            _w(f, [len(wused)], np.int32)
+            _w(f, wused, np.int32)
+            for iwb in wused:
+                _w(f, chi_loc_fn(iwb, "dens"), np.complex128)
+                _w(f, chi_loc_fn(iwb, "magn"), np.complex128)
+        env = dict(os.environ)
+        env.update(MKL_NUM_THREADS=str(threads), MKL_DYNAMIC="FALSE", OMP_NUM_THREADS=str(threads))
+        import time
+        t0 = time.perf_counter()
+        subprocess.run([HARNESS, fi, fo], check=True, env=env, timeout=timeout)
+        dt = time.perf_counter() - t0
+        raw = np.fromfile(fo, dtype=np.complex128)
+    shapes = [("chi_d", (n2, n2, npts)), ("chi_m", (n2, n2, npts)), ("bubble_nl", (n2, n2, npts)),
+              ("sigma_nl", (n, n, nf, nkp_eom)), ("sigma_dmft", (n, n, nf)),
+              ("chi_loc_d", (n2, n2, nw)), ("chi_loc_m", (n2, n2, nw)), ("bubble_loc", (n2, n2, nw)),
+              ("gammaqd0", (n2, N)), ("eta_d0", (n2, N)), ("eta_m0", (n2, N))]
+    out = {}
+    o = 0
+    for name, shp in shapes:
+        sz = int(np.prod(shp))
+        out[name] = raw[o:o + sz].reshape(shp, order="F")
+        o += sz
+    assert o == raw.size, (o, raw.size)
+    out["wall_s"] = dt
+    return out

@@ -1,0 +1,151 @@
+"""Parity of the HIP path (through the C ABI) with the reference.
+
+* golden: the reference's own Fortran modules' outputs (tests/golden) on small workloads, every
+  accumulated quantity, both precision modes.  Bound 1e-10 relative (max-abs normalised) in FP64
+  and in refined mode (complex64 LU + FP64 refinement to a relative residual <= 1e-13).
+* full size: the 3-orbital t2g workload at 20^3 k, 2x60 x 31 frequencies (BSE N = 1080) against
+  the numpy oracle on sampled points, and the F = 0 identity at that size.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err
+from adga_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_REFINED = 1e-10
+
+
+def _api():
+    from adga_amd import api
+    return api
+
+
+def _compare_golden(out, gold, wl, tol):
+    for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+        e = rel_err(out[k], gold[k])
+        assert e < tol, (k, e)
+    for w in out["chi_loc_d"]:
+        i = w + wl.iwbmax_small
+        assert rel_err(out["chi_loc_d"][w], gold["chi_loc_d"][:, :, i]) < tol
+        assert rel_err(out["chi_loc_m"][w], gold["chi_loc_m"][:, :, i]) < tol
+        assert rel_err(out["bubble_loc"][w], gold["bubble_loc"][:, :, i]) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "refined"])
+@pytest.mark.parametrize("name", ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8"])
+def test_gpu_matches_reference_golden(name, precision):
+    from golden.make_golden import FIXTURES
+    api = _api()
+    gold = load_golden(name)
+    wl = synth.workload(FIXTURES[name][0])
+    inp = synth.build_inputs(wl)
+    pts = [tuple(p) for p in gold["points"].tolist()]
+    print(f"case {name} {precision} points={len(pts)}", flush=True)
+    with api.Context(inp, batch=512) as ctx:
+        out = api.run_loop(ctx, pts, lambda w, c: synth.chi_loc(inp, w, c), precision)
+        st = ctx.stats()
+    assert (out["status"] == 0).all()
+    _compare_golden(out, gold, wl, TOL_FP64 if precision == "fp64" else TOL_REFINED)
+    if precision == "refined":
+        assert st["refine_resid"] <= 1e-13
+
+
+def test_gpu_inverse_matrix_matches_numpy():
+    api = _api()
+    rng = np.random.default_rng(3)
+    for n in (1, 3, 40, 300):
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.eye(n)
+        inv, ierr = api.inverse_matrix(a)
+        assert ierr == 0
+        assert rel_err(inv, np.linalg.inv(a)) < 1e-11
+
+
+def test_gpu_inverse_matrix_singular_raises():
+    api = _api()
+    a = np.ones((8, 8), dtype=complex)
+    with pytest.raises(api.AdgaError) as e:
+        api.inverse_matrix(a)
+    assert e.value.code == 3  # ADGA_E_SINGULAR, the reference's "ERROR in ZGETRF"
+
+
+def test_gpu_nonfinite_vertex_is_reported():
+    api = _api()
+    wl = synth.workload("t2g_k4")
+    inp = synth.build_inputs(wl)
+    with api.Context(inp) as ctx:
+        xd = synth.chi_loc(inp, 0, "dens")
+        xd[3, 5] = np.nan
+        ctx.set_local_omega(0, xd, synth.chi_loc(inp, 0, "magn"))
+        r = ctx.run_points(np.array([1, 2, 3]), "fp64", raise_on_error=False)
+        assert r["rc"] != 0 and (r["status"] != 0).all()
+        r = ctx.run_points(np.array([1, 2]), "refined", raise_on_error=False)
+        assert r["rc"] != 0 and (r["status"] != 0).all()
+
+
+def test_gpu_state_and_argument_errors():
+    api = _api()
+    inp = synth.build_inputs(synth.workload("sc1_k4"))
+    with api.Context(inp) as ctx:
+        with pytest.raises(api.AdgaError) as e:
+            ctx.run_points(np.array([1]))
+        assert e.value.code == 6  # set_local_omega first
+        ctx.set_local_omega(0, synth.chi_loc(inp, 0, "dens"), synth.chi_loc(inp, 0, "magn"))
+        with pytest.raises(api.AdgaError) as e:
+            ctx.run_points(np.array([0]))
+        assert e.value.code == 1
+
+
+@pytest.fixture(scope="module")
+def t2g_full():
+    wl = synth.workload("t2g_k20")
+    return synth.build_inputs(wl)
+
+
+def test_gpu_full_size_against_oracle(t2g_full):
+    """3 orbitals, 20^3 k, N = 1080: two points at omega = 0 (one at Gamma, so the local EOM runs too)
+    and one at omega = 3, both precision modes, against the numpy oracle."""
+    import adga_oracle
+    api = _api()
+    inp = t2g_full
+    hp = adga_oracle.HotPath(inp)
+    pts = [(0, 0), (0, 1234), (3, 777)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    ref = hp.run(pts, cl)
+    for precision in ("fp64", "refined"):
+        print(f"full size {precision}", flush=True)
+        with api.Context(inp) as ctx:
+            out = api.run_loop(ctx, pts, cl, precision)
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+            e = rel_err(out[k], ref[k])
+            assert e < 1e-10, (precision, k, e)
+
+
+def test_gpu_full_size_F_zero(t2g_full):
+    """chi_loc = chi0^w at N = 1080: eta = 0, so the non-local chi parts and Sigma_nl vanish."""
+    import dataclasses
+    api = _api()
+    inp = dataclasses.replace(t2g_full, wl=dataclasses.replace(t2g_full.wl, noise=0.0))
+    pts = [(1, q) for q in range(0, 8000, 97)]
+    with api.Context(inp) as ctx:
+        out = api.run_loop(ctx, pts, lambda w, c: synth.chi_loc(inp, w, c), "refined")
+    scale = np.abs(out["bubble_nl"]).max()
+    assert np.abs(out["chi_d"]).max() < 1e-12 * scale
+    assert np.abs(out["chi_m"]).max() < 1e-12 * scale
+    assert np.abs(out["sigma_nl"]).max() < 1e-14
+
+
+def test_gpu_refined_equals_fp64_on_a_batch(t2g_full):
+    """A 96-point batch at omega = -2 in both modes: per-quantity agreement within 1e-10."""
+    api = _api()
+    inp = t2g_full
+    pts = [(-2, q) for q in range(0, 8000, 83)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    res = {}
+    for precision in ("fp64", "refined"):
+        with api.Context(inp) as ctx:
+            res[precision] = api.run_loop(ctx, pts, cl, precision)
+    for k in ("chi_d", "chi_m", "sigma_nl"):
+        assert rel_err(res["refined"][k], res["fp64"][k]) < 1e-10, k
