@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libadga_amd.so")
-SOURCES = ["local.hip", "bse.hip", "eom.hip", "api.cpp"]
+SOURCES = ["local.hip", "bse.hip", "lu.hip", "eom.hip", "api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -34,6 +34,32 @@ def _compile(src):
     return out
 
 
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FORTRAN = ["adga_c.f90", "lapack_module.f90", "adga_gpu.f90", "fortran_check.f90"]
+FCHECK = os.path.join(HERE, "fortran", "adga_fortran_check")
+
+
+def build_fortran() -> str | None:
+    """Fortran host layer (ISO_C_BINDING modules) and its check program, linked to the library."""
+    if not os.path.exists(FC):
+        return None
+    fdir = os.path.join(OBJ, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    objs = []
+    for src in FORTRAN:
+        out = os.path.join(fdir, os.path.splitext(src)[0] + ".o")
+        r = subprocess.run([FC, "-O2", "-fPIC", "-module-dir", fdir, "-c", os.path.join(HERE, "fortran", src),
+                            "-o", out], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stderr[-4000:]}")
+        objs.append(out)
+    r = subprocess.run([FC, "-o", FCHECK, *objs, "-L" + HERE, "-ladga_amd", "-Wl,-rpath,$ORIGIN/.."],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stderr[-4000:]}")
+    return FCHECK
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -44,8 +70,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    fexe = build_fortran()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", fexe or "(no amdflang: Fortran layer skipped)")
     return LIB
 
 

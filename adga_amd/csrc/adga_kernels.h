@@ -1,5 +1,7 @@
 // Host-side launchers of the gfx950 kernels (internal to libadga_amd.so).
 #pragma once
+#include <vector>
+
 #include "adga_common.h"
 
 namespace adga {
@@ -15,19 +17,17 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
                   const z64* opgm, const z64* B, cplx<T>* M, z64* gamq, hipStream_t s);
 template <typename T>
 int lu_leaf_width(int N);
+std::vector<int> lu_ops_factor(int N, int W, int lw);
+std::vector<int> lu_ops_solve(int N, int W, int lw);
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, cplx<T>* M, int* piv, int* info, hipStream_t s);
+bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, int nops, cplx<T>* M, int* piv, int* info,
+               hipStream_t s);
 template <typename T>
-bool launch_lu_apply(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* piv, cplx<T>* x,
-                     hipStream_t s);
-template <typename T>
-void launch_extract(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, z64* Y, hipStream_t s);
+void launch_rhs_io(int nmat, int N, int nrhs, int ldm, int stage, cplx<T>* M, z64* Y, const z64* R, hipStream_t s);
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
                 const z64* opgm, z64* Z, z64* B, hipStream_t s);
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
                      const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s);
-template <typename T>
-void launch_correct(long tot, const z64* R, cplx<T>* x, int stage, z64* Y, hipStream_t s);
 void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
                     const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
                     z64* chi_out, z64* bub_out, hipStream_t s);

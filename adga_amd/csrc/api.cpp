@@ -40,9 +40,12 @@ struct adga_ctx {
   z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
       *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
   void* M = nullptr;
-  void* xcor = nullptr;
   double* rel = nullptr;
   int *piv = nullptr, *info = nullptr, *qidx = nullptr;
+  // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
+  int* ops_f[2] = {nullptr, nullptr};
+  int* ops_s[2] = {nullptr, nullptr};
+  int nops_f[2] = {0, 0}, nops_s[2] = {0, 0};
   // accumulated self-energies
   z64 *sig_nl = nullptr, *sig_dmft = nullptr;
   double stats[11] = {0};  // see adga_get_stats
@@ -91,7 +94,7 @@ void upload(z64* d, const double* h, size_t count, hipStream_t s) {
 
 void free_batch(adga_ctx* c) {
   dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
-  dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->xcor); dfree(c->rel);
+  dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->rel);
   dfree(c->piv); dfree(c->info); dfree(c->qidx);
   c->cap = 0;
 }
@@ -114,7 +117,6 @@ void ensure_batch(adga_ctx* c, int nb) {
   dalloc(&c->chi_out, 2 * nb * (size_t)g.n4);
   dalloc(&c->bub_out, nb * (size_t)g.n4);
   dalloc((char**)&c->M, 2 * nb * (size_t)g.N * c->ldm * sizeof(z64));
-  dalloc((char**)&c->xcor, 2 * nb * pmat * sizeof(z64));
   dalloc(&c->rel, 2 * nb);
   dalloc(&c->piv, 2 * nb * (size_t)g.N);
   dalloc(&c->info, 2 * nb);
@@ -125,6 +127,25 @@ void ensure_batch(adga_ctx* c, int nb) {
 int fail(adga_ctx* c, int code, const char* msg) {
   if (c) c->err = msg;
   return code;
+}
+
+int* upload_ops(const std::vector<int>& v) {
+  int* d = nullptr;
+  dalloc(&d, v.size());
+  CK(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+// prec 0: complex128 LU, prec 1: complex64 LU
+void ensure_ops(adga_ctx* c, int prec) {
+  if (c->ops_f[prec]) return;
+  const int N = c->g.N, W = c->g.N + c->g.n2;
+  const int lw = prec == 0 ? adga::lu_leaf_width<double>(N) : adga::lu_leaf_width<float>(N);
+  std::vector<int> f = adga::lu_ops_factor(N, W, lw), so = adga::lu_ops_solve(N, W, lw);
+  c->ops_f[prec] = upload_ops(f);
+  c->ops_s[prec] = upload_ops(so);
+  c->nops_f[prec] = (int)(f.size() / 8);
+  c->nops_s[prec] = (int)(so.size() / 8);
 }
 
 }  // namespace
@@ -297,14 +318,16 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       int iters = 0;
       double maxrel = 0.0;
       bool lu_ok;
+      ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
       if (precision == ADGA_PREC_FP64) {
         adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (z64*)c->M, c->gamq, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
-        lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, (z64*)c->M, c->piv, c->info, s);
+        lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, c->ops_f[0], c->nops_f[0], (z64*)c->M, c->piv,
+                                        c->info, s);
         CK(hipEventRecord(lu1, s));
         if (lu_ok) {
-          adga::launch_extract<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->Y, s);
+          adga::launch_rhs_io<double>(nmat, g.N, n2, c->ldm, 0, (z64*)c->M, c->Y, nullptr, s);
           // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
           relh.resize(nmat);
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
@@ -317,9 +340,10 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (c32*)c->M, c->gamq, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
-        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, (c32*)c->M, c->piv, c->info, s);
+        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_f[1], c->nops_f[1], (c32*)c->M, c->piv, c->info,
+                                       s);
         CK(hipEventRecord(lu1, s));
-        if (lu_ok) adga::launch_extract<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->Y, s);
+        if (lu_ok) adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 0, (c32*)c->M, c->Y, nullptr, s);
         relh.resize(nmat);
         for (int it = 0; lu_ok; ++it) {
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
@@ -332,9 +356,11 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
             else maxrel = std::max(maxrel, r);
           }
           if (bad || maxrel <= c->refine_tol || it >= c->refine_maxit) break;
-          adga::launch_correct<float>(tot, c->R, (c32*)c->xcor, 0, c->Y, s);
-          adga::launch_lu_apply<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (c32*)c->xcor, s);
-          adga::launch_correct<float>(tot, c->R, (c32*)c->xcor, 1, c->Y, s);
+          // correction: the residual (rounded) replaces the right-hand-side columns, solve with the
+          // complex64 factors, add the correction to the FP64 solution
+          adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 2, (c32*)c->M, nullptr, c->R, s);
+          adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_s[1], c->nops_s[1], (c32*)c->M, c->piv, nullptr, s);
+          adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 1, (c32*)c->M, c->Y, nullptr, s);
           iters = it + 1;
         }
       }
@@ -440,6 +466,7 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->xd); dfree(c->xm); dfree(c->cd); dfree(c->cm); dfree(c->gamd); dfree(c->gamm); dfree(c->opgd);
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
   dfree(c->sig_nl); dfree(c->sig_dmft);
+  for (int p = 0; p < 2; ++p) { dfree(c->ops_f[p]); dfree(c->ops_s[p]); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ADGA_OK;
@@ -464,7 +491,9 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dalloc(&dpiv, (size_t)n);
     dalloc(&dinfo, 1);
     CK(hipMemcpy(dM, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
-    bool ok = adga::launch_lu<double>(1, n, W, ldm, dM, dpiv, dinfo, nullptr);
+    std::vector<int> ops = adga::lu_ops_factor(n, W, adga::lu_leaf_width<double>(n));
+    int* dops = upload_ops(ops);
+    bool ok = adga::launch_lu<double>(1, n, W, ldm, dops, (int)(ops.size() / 8), dM, dpiv, dinfo, nullptr);
     if (ok) {
       CK(hipGetLastError());
       CK(hipMemcpy(h.data(), dM, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
@@ -473,6 +502,7 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dfree(dM);
     dfree(dpiv);
     dfree(dinfo);
+    dfree(dops);
     if (!ok) return ADGA_E_ARG;
     if (*ierr != 0) return ADGA_E_SINGULAR;
     z64* dst = reinterpret_cast<z64*>(m);

@@ -1,5 +1,5 @@
-// A3 (bubble), A6 (BSE build), A7/A8 (batched complex LU + solve, FP64 or
-// complex64 with FP64 iterative refinement) on gfx950.
+// A3 (bubble), A6 (BSE build) and the FP64 right-hand side / residuals of the A7/A8 solves
+// (the LU itself is in lu.hip) on gfx950.
 //
 // Formulation (main.F90:673-757): for each channel r the reference forms
 // A = 1 - blockdiag(S_nu) C with S_nu = chi0^q(nu) diag(chi0^w(nu))^-1 - 1 and
@@ -148,382 +148,7 @@ template void launch_build<float>(const Geom&, int, int, const z64*, const z64*,
 template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, const z64*,
                                    const z64*, z64*, z64*, hipStream_t);
 
-// ---------------------------------------------------------------- A7: batched LU + solve
-// One workgroup (256 threads) per matrix.  For each leaf of LW columns:
-//   1. the leaf (rows k0..N-1) is held in registers (RPT rows per thread) and factored column by
-//      column with partial pivoting (|re|+|im| like izamax, first index on ties);
-//   2. the leaf's row interchanges are applied to columns right of the leaf, and the leaf's rows of
-//      those columns are solved with the unit-lower L11 (one thread per column);
-//   3. the trailing matrix (including the right-hand-side columns) is updated, C -= L21 U12.
-// Columns left of the leaf are not permuted: L is kept in elimination order, which is the order
-// the refinement solve applies it in.  Finally U Y = Z is solved for the right-hand sides.
-constexpr int LU_NT = 256;
-
-template <typename T, int RPT, int LW>
-__global__ void __launch_bounds__(LU_NT) k_lu(int N, int W, int ldm, cplx<T>* __restrict__ Mall,
-                                              int* __restrict__ pivall, int* __restrict__ info) {
-  using C = cplx<T>;
-  constexpr int NWAVE = LU_NT / 64;
-  constexpr int TS = 64;  // GEMM tile
-  __shared__ C sP[LW], sQ[LW];
-  __shared__ C sL[LW][LW + 1];
-  __shared__ T redv[NWAVE];
-  __shared__ int redi[NWAVE];
-  __shared__ int spiv[LW];
-  __shared__ C tA[TS][LW + 1];
-  __shared__ C tB[LW][TS + 1];
-  const int t = threadIdx.x;
-  const int lane = t % 64, wave = t / 64;
-  C* M = Mall + (long)blockIdx.x * N * ldm;
-  int* piv = pivall + (long)blockIdx.x * N;
-  int myinfo = 0;
-
-  for (int k0 = 0; k0 < N; k0 += LW) {
-    const int w = min(LW, N - k0);
-    C reg[RPT][LW];
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      int r = k0 + t + i * LU_NT;
-#pragma unroll
-      for (int j = 0; j < LW; ++j) reg[i][j] = (r < N && j < w) ? M[(long)r * ldm + k0 + j] : mk<T>(0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < LW; ++j) {
-      if (j < w) {
-        const int kc = k0 + j;
-        // ---- pivot search
-        T bv = (T)-1;
-        int br = 0x7fffffff;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-          int r = k0 + t + i * LU_NT;
-          if (r < N && r >= kc) {
-            T v = cabs1(reg[i][j]);
-            if (v > bv) { bv = v; br = r; }
-          }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          T ov = __shfl_xor(bv, off);
-          int orow = __shfl_xor(br, off);
-          if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
-        }
-        if (lane == 0) { redv[wave] = bv; redi[wave] = br; }
-        __syncthreads();
-        bv = redv[0];
-        br = redi[0];
-#pragma unroll
-        for (int q = 1; q < NWAVE; ++q) {
-          T ov = redv[q];
-          int orow = redi[q];
-          if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
-        }
-        // no finite candidate (NaN column): keep the diagonal row and flag the matrix
-        const int p = (br < N) ? br : kc;
-        if (br >= N && myinfo == 0) myinfo = kc + 1;
-        const int tp = (p - k0) % LU_NT, sp = (p - k0) / LU_NT;
-        // ---- row interchange inside the leaf through LDS
-        if (t == tp) {
-#pragma unroll
-          for (int i = 0; i < RPT; ++i)
-            if (i == sp) {
-#pragma unroll
-              for (int jj = 0; jj < LW; ++jj) sP[jj] = reg[i][jj];
-            }
-        }
-        if (p != kc && t == j) {
-#pragma unroll
-          for (int jj = 0; jj < LW; ++jj) sQ[jj] = reg[0][jj];
-        }
-        if (t == 0) { spiv[j] = p; piv[kc] = p; }
-        __syncthreads();
-        if (p != kc) {
-          if (t == tp) {
-#pragma unroll
-            for (int i = 0; i < RPT; ++i)
-              if (i == sp) {
-#pragma unroll
-                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = sQ[jj];
-              }
-          }
-          if (t == j) {
-#pragma unroll
-            for (int jj = 0; jj < LW; ++jj) reg[0][jj] = sP[jj];
-          }
-        }
-        // ---- scale and rank-1 update of the rows below
-        const C pv = sP[j];
-        if (pv.re == (T)0 && pv.im == (T)0) {
-          if (myinfo == 0) myinfo = kc + 1;
-        } else {
-          const C rp = crecip(pv);
-#pragma unroll
-          for (int i = 0; i < RPT; ++i) {
-            int r = k0 + t + i * LU_NT;
-            if (r < N && r > kc) {
-              C l = cmul(reg[i][j], rp);
-              reg[i][j] = l;
-#pragma unroll
-              for (int jj = j + 1; jj < LW; ++jj) reg[i][jj] = cfms(l, sP[jj], reg[i][jj]);
-            }
-          }
-        }
-      }
-    }
-    // ---- store the leaf, keep L11 in LDS
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      int r = k0 + t + i * LU_NT;
-      if (r < N) {
-#pragma unroll
-        for (int j = 0; j < LW; ++j)
-          if (j < w) M[(long)r * ldm + k0 + j] = reg[i][j];
-      }
-    }
-    if (t < w) {
-#pragma unroll
-      for (int j = 0; j < LW; ++j) sL[t][j] = reg[0][j];
-    }
-    __syncthreads();
-    // ---- interchanges and U12 rows for the columns right of the leaf
-    for (int c = k0 + w + t; c < W; c += LU_NT) {
-      C x[LW];
-#pragma unroll
-      for (int j = 0; j < LW; ++j) {
-        if (j < w) {
-          int p = spiv[j];
-          long a0 = (long)(k0 + j) * ldm + c;
-          if (p != k0 + j) {
-            long a1 = (long)p * ldm + c;
-            C u = M[a0], v = M[a1];
-            M[a1] = u;
-            x[j] = v;
-          } else {
-            x[j] = M[a0];
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < LW; ++j) {
-        if (j < w) {
-#pragma unroll
-          for (int i = 0; i < j; ++i) x[j] = cfms(sL[j][i], x[i], x[j]);
-          M[(long)(k0 + j) * ldm + c] = x[j];
-        }
-      }
-    }
-    __syncthreads();
-    // ---- trailing update C -= L21 U12 over 64x64 tiles (4x4 complex outputs per thread)
-    const int r0 = k0 + w, c0 = k0 + w;
-    const int ty = t / 16, tx = t % 16;
-    for (int rt = r0; rt < N; rt += TS) {
-      for (int ct = c0; ct < W; ct += TS) {
-        for (int e = t; e < TS * LW; e += LU_NT) {
-          int rr = e / LW, j = e % LW;
-          int r = rt + rr;
-          tA[rr][j] = (r < N && j < w) ? M[(long)r * ldm + k0 + j] : mk<T>(0, 0);
-        }
-        for (int e = t; e < TS * LW; e += LU_NT) {
-          int j = e / TS, cc = e % TS;
-          int c = ct + cc;
-          tB[j][cc] = (c < W && j < w) ? M[(long)(k0 + j) * ldm + c] : mk<T>(0, 0);
-        }
-        __syncthreads();
-        C acc[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int bq = 0; bq < 4; ++bq) {
-            int r = rt + ty * 4 + a, c = ct + tx * 4 + bq;
-            acc[a][bq] = (r < N && c < W) ? M[(long)r * ldm + c] : mk<T>(0, 0);
-          }
-#pragma unroll
-        for (int j = 0; j < LW; ++j) {
-          C av[4], bvv[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) av[a] = tA[ty * 4 + a][j];
-#pragma unroll
-          for (int bq = 0; bq < 4; ++bq) bvv[bq] = tB[j][tx * 4 + bq];
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int bq = 0; bq < 4; ++bq) acc[a][bq] = cfms(av[a], bvv[bq], acc[a][bq]);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int bq = 0; bq < 4; ++bq) {
-            int r = rt + ty * 4 + a, c = ct + tx * 4 + bq;
-            if (r < N && c < W) M[(long)r * ldm + c] = acc[a][bq];
-          }
-        __syncthreads();
-      }
-    }
-  }
-  // ---- back substitution U Y = Z for the right-hand-side columns [N, W)
-  const int nrhs = W - N;
-  constexpr int BS = 32;
-  for (int i1 = N; i1 > 0; i1 -= BS) {
-    const int i0 = max(0, i1 - BS);
-    for (int pq = t; pq < (i1 - i0) * nrhs; pq += LU_NT) {
-      int i = i0 + pq / nrhs, s = pq % nrhs;
-      C acc = M[(long)i * ldm + N + s];
-      for (int j = i1; j < N; ++j) acc = cfms(M[(long)i * ldm + j], M[(long)j * ldm + N + s], acc);
-      M[(long)i * ldm + N + s] = acc;
-    }
-    __syncthreads();
-    for (int s = t; s < nrhs; s += LU_NT) {
-      for (int i = i1 - 1; i >= i0; --i) {
-        C acc = M[(long)i * ldm + N + s];
-        for (int j = i + 1; j < i1; ++j) acc = cfms(M[(long)i * ldm + j], M[(long)j * ldm + N + s], acc);
-        M[(long)i * ldm + N + s] = cdiv(acc, M[(long)i * ldm + i]);
-      }
-    }
-    __syncthreads();
-  }
-  if (t == 0) info[blockIdx.x] = myinfo;
-}
-
-// Solve M x = rhs with the factors of k_lu (same leaf width LW): forward elimination leaf by
-// leaf (interchanges, L11, L21), then back substitution with U.  x is overwritten in place
-// (row-major N x nrhs, type T).  One workgroup per matrix.
-template <typename T, int LW>
-__global__ void __launch_bounds__(LU_NT) k_lu_apply(int N, int nrhs, int ldm, const cplx<T>* __restrict__ Mall,
-                                                    const int* __restrict__ pivall, cplx<T>* __restrict__ xall) {
-  using C = cplx<T>;
-  const int t = threadIdx.x;
-  const C* M = Mall + (long)blockIdx.x * N * ldm;
-  const int* piv = pivall + (long)blockIdx.x * N;
-  C* x = xall + (long)blockIdx.x * N * nrhs;
-  for (int k0 = 0; k0 < N; k0 += LW) {
-    const int w = min(LW, N - k0);
-    for (int s = t; s < nrhs; s += LU_NT) {
-      for (int j = 0; j < w; ++j) {
-        int p = piv[k0 + j];
-        if (p != k0 + j) {
-          C u = x[(long)(k0 + j) * nrhs + s];
-          x[(long)(k0 + j) * nrhs + s] = x[(long)p * nrhs + s];
-          x[(long)p * nrhs + s] = u;
-        }
-      }
-      for (int j = 1; j < w; ++j) {
-        C acc = x[(long)(k0 + j) * nrhs + s];
-        for (int i = 0; i < j; ++i) acc = cfms(M[(long)(k0 + j) * ldm + k0 + i], x[(long)(k0 + i) * nrhs + s], acc);
-        x[(long)(k0 + j) * nrhs + s] = acc;
-      }
-    }
-    __syncthreads();
-    for (int pq = t; pq < (N - k0 - w) * nrhs; pq += LU_NT) {
-      int r = k0 + w + pq / nrhs, s = pq % nrhs;
-      C acc = x[(long)r * nrhs + s];
-      for (int j = 0; j < w; ++j) acc = cfms(M[(long)r * ldm + k0 + j], x[(long)(k0 + j) * nrhs + s], acc);
-      x[(long)r * nrhs + s] = acc;
-    }
-    __syncthreads();
-  }
-  constexpr int BS = 32;
-  for (int i1 = N; i1 > 0; i1 -= BS) {
-    const int i0 = max(0, i1 - BS);
-    for (int pq = t; pq < (i1 - i0) * nrhs; pq += LU_NT) {
-      int i = i0 + pq / nrhs, s = pq % nrhs;
-      C acc = x[(long)i * nrhs + s];
-      for (int j = i1; j < N; ++j) acc = cfms(M[(long)i * ldm + j], x[(long)j * nrhs + s], acc);
-      x[(long)i * nrhs + s] = acc;
-    }
-    __syncthreads();
-    for (int s = t; s < nrhs; s += LU_NT) {
-      for (int i = i1 - 1; i >= i0; --i) {
-        C acc = x[(long)i * nrhs + s];
-        for (int j = i + 1; j < i1; ++j) acc = cfms(M[(long)i * ldm + j], x[(long)j * nrhs + s], acc);
-        x[(long)i * nrhs + s] = cdiv(acc, M[(long)i * ldm + i]);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Leaf width and rows-per-thread chosen so that a leaf fits in registers.
-template <typename T, int RPT>
-struct LeafWidth {
-  static constexpr int value = sizeof(T) == 4 ? (RPT <= 5 ? 8 : (RPT <= 8 ? 8 : 4)) : (RPT <= 5 ? 8 : (RPT <= 8 ? 4 : 2));
-};
-
-static int round_rpt(int rpt) {
-  static const int opts[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
-  for (int o : opts)
-    if (rpt <= o) return o;
-  return -1;
-}
-
-template <typename T>
-int lu_leaf_width(int N) {
-  int rpt = round_rpt((N + LU_NT - 1) / LU_NT);
-  if (sizeof(T) == 4) return rpt <= 8 ? 8 : 4;
-  return rpt <= 5 ? 8 : (rpt <= 8 ? 4 : 2);
-}
-template int lu_leaf_width<float>(int);
-template int lu_leaf_width<double>(int);
-
-template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, cplx<T>* M, int* piv, int* info, hipStream_t s) {
-  int rpt = round_rpt((N + LU_NT - 1) / LU_NT);
-  dim3 grid(nmat), blk(LU_NT);
-#define ADGA_LU_CASE(R)                                                                                         \
-  case R:                                                                                                       \
-    hipLaunchKernelGGL((k_lu<T, R, LeafWidth<T, R>::value>), grid, blk, 0, s, N, W, ldm, M, piv, info); \
-    return true;
-  switch (rpt) {
-    ADGA_LU_CASE(1)
-    ADGA_LU_CASE(2)
-    ADGA_LU_CASE(3)
-    ADGA_LU_CASE(4)
-    ADGA_LU_CASE(5)
-    ADGA_LU_CASE(6)
-    ADGA_LU_CASE(8)
-    ADGA_LU_CASE(12)
-    ADGA_LU_CASE(16)
-    default: return false;
-  }
-#undef ADGA_LU_CASE
-}
-template bool launch_lu<float>(int, int, int, int, c32*, int*, int*, hipStream_t);
-template bool launch_lu<double>(int, int, int, int, z64*, int*, int*, hipStream_t);
-
-template <typename T>
-bool launch_lu_apply(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* piv, cplx<T>* x,
-                     hipStream_t s) {
-  int lw = lu_leaf_width<T>(N);
-  dim3 grid(nmat), blk(LU_NT);
-  switch (lw) {
-    case 16: hipLaunchKernelGGL((k_lu_apply<T, 16>), grid, blk, 0, s, N, nrhs, ldm, M, piv, x); return true;
-    case 8: hipLaunchKernelGGL((k_lu_apply<T, 8>), grid, blk, 0, s, N, nrhs, ldm, M, piv, x); return true;
-    case 4: hipLaunchKernelGGL((k_lu_apply<T, 4>), grid, blk, 0, s, N, nrhs, ldm, M, piv, x); return true;
-    case 2: hipLaunchKernelGGL((k_lu_apply<T, 2>), grid, blk, 0, s, N, nrhs, ldm, M, piv, x); return true;
-    default: return false;
-  }
-}
-template bool launch_lu_apply<float>(int, int, int, int, const c32*, const int*, c32*, hipStream_t);
-template bool launch_lu_apply<double>(int, int, int, int, const z64*, const int*, z64*, hipStream_t);
-
-// ---------------------------------------------------------------- solution extraction / refinement
-// Y[m][r][s] (FP64) = M_m[r][N + s]
-template <typename T>
-__global__ void k_extract(int N, int nrhs, int ldm, int nmat, const cplx<T>* __restrict__ M, z64* __restrict__ Y) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long tot = (long)nmat * N * nrhs;
-  if (idx >= tot) return;
-  int s = (int)(idx % nrhs);
-  long mr = idx / nrhs;  // m*N + r
-  Y[idx] = ccast<double>(M[mr * ldm + N + s]);
-}
-template <typename T>
-void launch_extract(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, z64* Y, hipStream_t s) {
-  long tot = (long)nmat * N * nrhs;
-  hipLaunchKernelGGL(k_extract<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nrhs, ldm, nmat, M, Y);
-}
-template void launch_extract<float>(int, int, int, int, const c32*, z64*, hipStream_t);
-template void launch_extract<double>(int, int, int, int, const z64*, z64*, hipStream_t);
-
+// ---------------------------------------------------------------- A7/A8 refinement residuals
 // Z[m][d*n2 + i'][s] = sum_c' S_b[d][c'][i'] V[m][d*n2 + c'][s]   (m = 2b + ch), i.e. Z = blockdiag(S)^T V,
 // with V = Y (a solution) or, when Y is null, V = (1 + gamma^w_ch)^T.
 __global__ void k_resid_z(Geom g, int nmat, const z64* __restrict__ S, const z64* __restrict__ Y,
@@ -637,18 +262,6 @@ __global__ void __launch_bounds__(256) k_resid_norm(Geom g, const z64* __restric
   if (t == 0) rel[m] = sb[0] > 0.0 ? sr[0] / sb[0] : (sr[0] > 0.0 ? INFINITY : 0.0);
 }
 
-// x (type T) = R for matrices that are still refining; Y += x afterwards.
-template <typename T>
-__global__ void k_round(long tot, const z64* __restrict__ R, cplx<T>* __restrict__ x) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < tot) x[idx] = ccast<T>(R[idx]);
-}
-template <typename T>
-__global__ void k_accum(long tot, const cplx<T>* __restrict__ x, z64* __restrict__ Y) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < tot) Y[idx] = cadd(Y[idx], ccast<double>(x[idx]));
-}
-
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.
 // B = M eta^T with B = C^T (blockdiag(S)^T (1+gamma^w)^T), in FP64.
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
@@ -674,16 +287,5 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
   hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 1, cm, B, Y, Z, R);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);
 }
-
-template <typename T>
-void launch_correct(long tot, const z64* R, cplx<T>* x, int stage, z64* Y, hipStream_t s) {
-  dim3 grid((unsigned)((tot + 255) / 256));
-  if (stage == 0)
-    hipLaunchKernelGGL(k_round<T>, grid, dim3(256), 0, s, tot, R, x);
-  else
-    hipLaunchKernelGGL(k_accum<T>, grid, dim3(256), 0, s, tot, x, Y);
-}
-template void launch_correct<float>(long, const z64*, c32*, int, z64*, hipStream_t);
-template void launch_correct<double>(long, const z64*, z64*, int, z64*, hipStream_t);
 
 }  // namespace adga

@@ -1,0 +1,94 @@
+! ISO_C_BINDING interfaces to libadga_amd.so (include/adga_amd.h).
+!
+! Complex arrays cross the boundary as complex(c_double_complex), which has the
+! layout of the reference's complex(kind=8): interleaved (re, im), column-major.
+module adga_c
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+  public :: adga_dims, c_adga_ctx_create, c_adga_set_option, c_adga_set_local_omega, c_adga_run_points, &
+            c_adga_get_sigma, c_adga_reset_sigma, c_adga_get_stats, c_adga_last_error, c_adga_ctx_destroy, &
+            c_adga_inverse_matrix_z, ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
+
+  integer(c_int), parameter :: ADGA_OK = 0, ADGA_E_SINGULAR = 3
+  integer(c_int), parameter :: ADGA_PREC_FP64 = 0, ADGA_PREC_REFINED = 1
+
+  type, bind(c) :: adga_dims
+     integer(c_int) :: ndim, nkpx, nkpy, nkpz, iwmax, iwfmax_small, iwbmax_small, nqp, nkp_eom
+     real(c_double) :: beta, mu
+  end type adga_dims
+
+  interface
+     integer(c_int) function c_adga_ctx_create(dims, q_data, k_data_eom, hk, siw, giw, dc, u, u_tilde, n_dmft, ctx) &
+          bind(c, name='adga_ctx_create')
+       import :: c_int, c_ptr, c_double_complex, adga_dims
+       type(adga_dims), intent(in) :: dims
+       integer(c_int), intent(in) :: q_data(*), k_data_eom(*)
+       complex(c_double_complex), intent(in) :: hk(*), siw(*), giw(*), dc(*), u(*), u_tilde(*), n_dmft(*)
+       type(c_ptr), intent(out) :: ctx
+     end function c_adga_ctx_create
+
+     integer(c_int) function c_adga_set_option(ctx, key, value) bind(c, name='adga_set_option')
+       import :: c_int, c_ptr, c_char, c_double
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: key(*)
+       real(c_double), value :: value
+     end function c_adga_set_option
+
+     integer(c_int) function c_adga_set_local_omega(ctx, iwb, chi_loc_dens, chi_loc_magn, chi_loc_dens_out, &
+          chi_loc_magn_out, bubble_loc_out) bind(c, name='adga_set_local_omega')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iwb
+       complex(c_double_complex), intent(in) :: chi_loc_dens(*), chi_loc_magn(*)
+       complex(c_double_complex), intent(out) :: chi_loc_dens_out(*), chi_loc_magn_out(*), bubble_loc_out(*)
+     end function c_adga_set_local_omega
+
+     integer(c_int) function c_adga_run_points(ctx, npts, iq, precision, chi_dens_out, chi_magn_out, bubble_nl_out, &
+          status) bind(c, name='adga_run_points')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: npts, precision
+       integer(c_int), intent(in) :: iq(*)
+       complex(c_double_complex), intent(out) :: chi_dens_out(*), chi_magn_out(*), bubble_nl_out(*)
+       integer(c_int), intent(out) :: status(*)
+     end function c_adga_run_points
+
+     integer(c_int) function c_adga_get_sigma(ctx, sigma_nl, sigma_dmft, sigma_hf) bind(c, name='adga_get_sigma')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: sigma_nl(*), sigma_dmft(*), sigma_hf(*)
+     end function c_adga_get_sigma
+
+     integer(c_int) function c_adga_reset_sigma(ctx) bind(c, name='adga_reset_sigma')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function c_adga_reset_sigma
+
+     integer(c_int) function c_adga_get_stats(ctx, out, nout) bind(c, name='adga_get_stats')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(*)
+       integer(c_int), value :: nout
+     end function c_adga_get_stats
+
+     integer(c_int) function c_adga_last_error(ctx, buf, len) bind(c, name='adga_last_error')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int), value :: len
+     end function c_adga_last_error
+
+     integer(c_int) function c_adga_ctx_destroy(ctx) bind(c, name='adga_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function c_adga_ctx_destroy
+
+     integer(c_int) function c_adga_inverse_matrix_z(n, m, ierr) bind(c, name='adga_inverse_matrix_z')
+       import :: c_int, c_double_complex
+       integer(c_int), value :: n
+       complex(c_double_complex), intent(inout) :: m(*)
+       integer(c_int), intent(out) :: ierr
+     end function c_adga_inverse_matrix_z
+  end interface
+end module adga_c

@@ -25,9 +25,10 @@ def _w(f, a, dtype):
     np.asarray(a, dtype=dtype).flatten(order="F").tofile(f)
 
 
-def run(inp, points, chi_loc_fn, threads=1, timeout=3600):
+def run(inp, points, chi_loc_fn, threads=1, timeout=3600, exe=None, args=()):
     """Run the reference modules over `points` = [(iwb, iq0), ...] (iq0 0-based).
-    Returns a dict of Fortran-shaped complex128 arrays and the wall time."""
+    Returns a dict of Fortran-shaped complex128 arrays and the wall time.  `exe` runs another
+    program that reads and writes the same file layout (the GPU path's Fortran host check)."""
     wl = inp.wl
     n, n2, nf = wl.n, wl.n * wl.n, wl.nf
     N = n2 * nf
@@ -57,7 +58,7 @@ def run(inp, points, chi_loc_fn, threads=1, timeout=3600):
         env.update(MKL_NUM_THREADS=str(threads), MKL_DYNAMIC="FALSE", OMP_NUM_THREADS=str(threads))
         import time
         t0 = time.perf_counter()
-        subprocess.run([HARNESS, fi, fo], check=True, env=env, timeout=timeout)
+        subprocess.run([exe or HARNESS, fi, fo, *args], check=True, env=env, timeout=timeout)
         dt = time.perf_counter() - t0
         raw = np.fromfile(fo, dtype=np.complex128)
     shapes = [("chi_d", (n2, n2, npts)), ("chi_m", (n2, n2, npts)), ("bubble_nl", (n2, n2, npts)),
