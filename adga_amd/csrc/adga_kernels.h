@@ -13,8 +13,9 @@ void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z
 void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s);
 void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64* S, hipStream_t s);
 template <typename T>
-void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
-                  const z64* opgm, const z64* B, cplx<T>* M, z64* gamq, hipStream_t s);
+void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* B,
+                  cplx<T>* M, hipStream_t s);
+void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s);
 template <typename T>
 int lu_leaf_width(int N);
 std::vector<int> lu_ops_factor(int N, int W, int lw);
@@ -31,10 +32,20 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
 void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
                     const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
                     z64* chi_out, z64* bub_out, hipStream_t s);
-void launch_eom(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* u, const z64* ut,
-                const z64* eta, const z64* gamq, z64* mt, const z64* gt, z64* sigma, hipStream_t s);
 void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
                       const z64* ndmft, z64* sdmft, hipStream_t s);
 void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s);
+void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
+                 hipStream_t s);
+void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,
+                       const z64* gt, z64* sigma, hipStream_t s);
+// k-grid convolutions (kconv.hip)
+void dft3(const Geom& g, long nfield, int sign, const z64* tw, z64* a, z64* b, hipStream_t s);
+void launch_ghat(const Geom& g, const z64* tw, const z64* gt, z64* tmp, z64* ghat, hipStream_t s);
+void launch_bubble_all(const Geom& g, int iwb, const z64* tw, const z64* ghat, z64* P, z64* out, hipStream_t s);
+void launch_bubble_gather(const Geom& g, int npts, const int* qidx, const z64* all, z64* chi0q, hipStream_t s);
+void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s);
+void launch_eom_flush(const Geom& g, int iwb, const int* keom, const z64* tw, const z64* ghat, z64* macc, z64* work,
+                      z64* sigma, hipStream_t s);
 
 }  // namespace adga

@@ -48,6 +48,14 @@ struct adga_ctx {
   int nops_f[2] = {0, 0}, nops_s[2] = {0, 0};
   // accumulated self-energies
   z64 *sig_nl = nullptr, *sig_dmft = nullptr;
+  // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
+  int bubble_mode = 2;  // 0 direct, 1 DFT, 2 DFT when the batch has >= 64 points or the cache is valid
+  int eom_mode = 1;     // 0 direct, 1 DFT (accumulated per omega, flushed on omega change / get_sigma)
+  z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
+  bool bub_valid = false;
+  int bub_iwb = 0;
+  bool eom_pending = false;
+  int eom_iwb = 0;
   double stats[11] = {0};  // see adga_get_stats
 };
 
@@ -107,7 +115,7 @@ void ensure_batch(adga_ctx* c, int nb) {
   size_t pmat = (size_t)g.N * g.n2;
   dalloc(&c->chi0q, nb * per_pt);
   dalloc(&c->S, nb * per_pt);
-  dalloc(&c->gamq, nb * pmat);
+  dalloc(&c->gamq, 2 * nb * pmat);
   dalloc(&c->Bp, 2 * nb * pmat);
   dalloc(&c->Y, 2 * nb * pmat);
   dalloc(&c->Z, 2 * nb * pmat);
@@ -127,6 +135,37 @@ void ensure_batch(adga_ctx* c, int nb) {
 int fail(adga_ctx* c, int code, const char* msg) {
   if (c) c->err = msg;
   return code;
+}
+
+void ensure_conv(adga_ctx* c) {
+  const Geom& g = c->g;
+  if (c->tw) return;
+  // forward twiddles e^{-2 pi i R k / n} of the three axes
+  std::vector<z64> tw;
+  for (int n : {g.nkx, g.nky, g.nkz})
+    for (int R = 0; R < n; ++R)
+      for (int k = 0; k < n; ++k) {
+        double ph = -2.0 * M_PI * (double)((long)R * k % n) / n;
+        tw.push_back(adga::mk(cos(ph), sin(ph)));
+      }
+  dalloc(&c->tw, tw.size());
+  CK(hipMemcpy(c->tw, tw.data(), sizeof(z64) * tw.size(), hipMemcpyHostToDevice));
+  size_t ng = (size_t)g.nw * g.n2 * g.nkp;
+  z64* tmp = nullptr;
+  dalloc(&tmp, ng);
+  dalloc(&c->ghat, ng);
+  adga::launch_ghat(g, c->tw, c->gt, tmp, c->ghat, c->stream);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->stream));
+  dfree(tmp);
+}
+
+void eom_flush(adga_ctx* c) {
+  if (!c->eom_pending) return;
+  adga::launch_eom_flush(c->g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream);
+  CK(hipGetLastError());
+  CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)c->g.iwf * c->g.n4 * c->g.nkp, c->stream));
+  c->eom_pending = false;
 }
 
 int* upload_ops(const std::vector<int>& v) {
@@ -251,6 +290,18 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
     c->refine_maxit = (int)v;
     return ADGA_OK;
   }
+  if (!strcmp(key, "bubble")) {
+    if (!(v == 0 || v == 1 || v == 2)) return fail(c, ADGA_E_ARG, "bubble must be 0 (direct), 1 (dft) or 2 (auto)");
+    c->bubble_mode = (int)v;
+    return ADGA_OK;
+  }
+  if (!strcmp(key, "eom")) {
+    if (!(v == 0 || v == 1)) return fail(c, ADGA_E_ARG, "eom must be 0 (direct) or 1 (dft)");
+    if ((int)v != c->eom_mode && c->eom_pending)
+      return fail(c, ADGA_E_STATE, "eom mode cannot change while an omega is pending (call adga_get_sigma)");
+    c->eom_mode = (int)v;
+    return ADGA_OK;
+  }
   return fail(c, ADGA_E_ARG, "unknown option");
 }
 
@@ -261,6 +312,8 @@ int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* x
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     hipStream_t s = c->stream;
+    if (c->eom_pending && c->eom_iwb != iwb) eom_flush(c);
+    if (c->bub_iwb != iwb) c->bub_valid = false;
     size_t NN = (size_t)g.N * g.N;
     upload(c->xd, xd, NN, s);
     upload(c->xm, xm, NN, s);
@@ -311,8 +364,25 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         if (iq[p0 + b] == 1)
           adga::launch_eom_local(g, c->iwb, c->iwb == 0, c->u, c->gamd, c->giw, c->ndmft, c->sig_dmft, s);
       CK(hipEventRecord(ev[1], s));
-      adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->chi0q, s);
+      const bool use_dft = c->bubble_mode == 1 || (c->bubble_mode == 2 && ((c->bub_valid && c->bub_iwb == c->iwb) || nb >= 64));
+      if (use_dft) {
+        ensure_conv(c);
+        if (!(c->bub_valid && c->bub_iwb == c->iwb)) {
+          size_t nb_all = (size_t)g.nf * g.n4 * g.nkp;
+          if (!c->bub_p) {
+            dalloc(&c->bub_p, nb_all);
+            dalloc(&c->bub_all, nb_all);
+          }
+          adga::launch_bubble_all(g, c->iwb, c->tw, c->ghat, c->bub_p, c->bub_all, s);
+          c->bub_valid = true;
+          c->bub_iwb = c->iwb;
+        }
+        adga::launch_bubble_gather(g, nb, c->qidx, c->bub_all, c->chi0q, s);
+      } else {
+        adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->chi0q, s);
+      }
       adga::launch_smat(g, nb, c->chi0q, c->winv, c->S, s);
+      adga::launch_gamq(g, nb, c->S, c->cd, c->Z, c->gamq, s);
       adga::launch_rhs(g, nb, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Z, c->Bp, s);
       CK(hipEventRecord(ev[2], s));
       int iters = 0;
@@ -320,7 +390,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       bool lu_ok;
       ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
       if (precision == ADGA_PREC_FP64) {
-        adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (z64*)c->M, c->gamq, s);
+        adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, c->Bp, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, c->ops_f[0], c->nops_f[0], (z64*)c->M, c->piv,
@@ -337,7 +407,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
             if (std::isfinite(r)) maxrel = std::max(maxrel, r);
         }
       } else {
-        adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Bp, (c32*)c->M, c->gamq, s);
+        adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->Bp, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
         lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_f[1], c->nops_f[1], (c32*)c->M, c->piv, c->info,
@@ -370,7 +440,22 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       adga::launch_eta_chi(g, nb, c->Y, c->opgd, c->opgm, c->eta, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc,
                            c->chi_out, c->bub_out, s);
       CK(hipEventRecord(ev[5], s));
-      adga::launch_eom(g, c->iwb, nb, c->qidx, c->keom, c->u, c->ut, c->eta, c->gamq, c->mt, c->gt, c->sig_nl, s);
+      adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s);
+      if (c->eom_mode == 1) {
+        ensure_conv(c);
+        if (!c->macc) {
+          size_t nm = (size_t)g.iwf * g.n4 * g.nkp;
+          dalloc(&c->macc, nm);
+          dalloc(&c->mwork, nm);
+          CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, s));
+        }
+        if (c->eom_pending && c->eom_iwb != c->iwb) eom_flush(c);
+        adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s);
+        c->eom_pending = true;
+        c->eom_iwb = c->iwb;
+      } else {
+        adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, c->gt, c->sig_nl, s);
+      }
       CK(hipEventRecord(ev[6], s));
       CK(hipGetLastError());
       chih.resize((size_t)2 * nb * g.n4);
@@ -425,6 +510,7 @@ int adga_get_sigma(adga_ctx* c, double* sig_nl, double* sig_dmft, double* sig_hf
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     hipStream_t s = c->stream;
+    eom_flush(c);
     adga::launch_fill_neg(g, g.nkp_eom, c->sig_nl, s);
     CK(hipGetLastError());
     if (sig_nl) CK(hipMemcpyAsync(sig_nl, c->sig_nl, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, hipMemcpyDeviceToHost, s));
@@ -441,6 +527,8 @@ int adga_reset_sigma(adga_ctx* c) {
     const Geom& g = c->g;
     CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, c->stream));
     CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, c->stream));
+    if (c->macc) CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp, c->stream));
+    c->eom_pending = false;
     CK(hipStreamSynchronize(c->stream));
     return ADGA_OK;
   });
@@ -467,6 +555,7 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
   dfree(c->sig_nl); dfree(c->sig_dmft);
   for (int p = 0; p < 2; ++p) { dfree(c->ops_f[p]); dfree(c->ops_s[p]); }
+  dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ADGA_OK;

@@ -95,58 +95,81 @@ void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64
 }
 
 // ---------------------------------------------------------------- A6: BSE build
-// Row r of M_{b,ch}: M(r, d*n2 + s) = delta - R_d(s, r), R_d(s, r) = sum_i' S_b[d][s][i'] C_ch(d*n2 + i', r);
-// M(r, N + s) = B(r, s), the right-hand side of M eta^T = B (launch_rhs).  For the density channel
-// gamma^q_d(s, r) = sum_d R_d(s, r) (main.F90:693-700).  One block per (row, channel, point).
-template <typename T>
+// M_{b,ch}(r, d*n2 + s) = delta - R_d(s, r),  R_d(s, r) = sum_i' S_b[d][s][i'] C_ch(d*n2 + i', r)
+// (main.F90:693-700, transposed: M = A^T); M(r, N + s) = B(r, s), the right-hand side of
+// M eta^T = B (launch_rhs); padding columns zero.  A workgroup owns DB frequency blocks d of one
+// (point, channel) for all rows: S_d sits in LDS, and each thread turns the n2 contiguous values
+// C(d*n2 + ., r) (a row slice of C^T) into n2 outputs.  The extra last workgroup along x writes
+// the right-hand-side and padding columns.
+template <typename T, int NN>
 __global__ void __launch_bounds__(256) k_build(Geom g, int ldm, const z64* __restrict__ S, const z64* __restrict__ c_d,
-                                               const z64* __restrict__ c_m, const z64* __restrict__ opg_d,
-                                               const z64* __restrict__ opg_m, const z64* __restrict__ B,
-                                               cplx<T>* __restrict__ M, z64* __restrict__ gamq) {
-  extern __shared__ z64 rrow[];  // N values of R for this row (density channel only)
-  const int r = blockIdx.x, ch = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
-  const int N = g.N, n2 = g.n2;
-  const z64* C = ch == 0 ? c_d : c_m;
-  const z64* Sb = S + (long)b * g.nf * g.n4;
-  const z64* crow = C + (long)r * N;  // C(., r) is contiguous (column r of the Fortran array)
-  cplx<T>* mrow = M + ((long)(2 * b + ch) * N + r) * ldm;
-  for (int c = t; c < ldm; c += 256) {
-    if (c < N) {
-      int d = c / n2, s = c % n2;
-      const z64* srow = Sb + (long)d * g.n4 + s * n2;
-      const z64* cc = crow + d * n2;
-      z64 acc = mk(0.0, 0.0);
-      for (int ip = 0; ip < n2; ++ip) acc = cfma(srow[ip], cc[ip], acc);
-      if (ch == 0) rrow[c] = acc;
-      z64 m = mk((c == r ? 1.0 : 0.0) - acc.re, -acc.im);
-      mrow[c] = ccast<T>(m);
-    } else if (c < N + n2) {
-      mrow[c] = ccast<T>(B[((long)(2 * b + ch) * N + r) * n2 + (c - N)]);
-    } else {
-      mrow[c] = mk<T>(0, 0);
+                                               const z64* __restrict__ c_m, const z64* __restrict__ B,
+                                               cplx<T>* __restrict__ M) {
+  constexpr int N2 = NN * NN, N4 = N2 * N2;
+  constexpr int DB = NN <= 3 ? 8 : 2;
+  __shared__ z64 sS[DB * N4];
+  const int ch = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
+  const int N = g.N;
+  cplx<T>* Mb = M + (long)(2 * b + ch) * N * ldm;
+  if ((int)blockIdx.x == (int)gridDim.x - 1) {  // right-hand side + padding columns
+    const int wcol = ldm - N;
+    const z64* Bm = B + (long)(2 * b + ch) * N * N2;
+    for (long e = t; e < (long)N * wcol; e += 256) {
+      int r = (int)(e / wcol), c = (int)(e % wcol);
+      Mb[(long)r * ldm + N + c] = c < N2 ? ccast<T>(Bm[(long)r * N2 + c]) : mk<T>(0, 0);
     }
+    return;
   }
-  if (ch == 0) {
-    __syncthreads();
-    if (t < n2) {
+  const int d0 = blockIdx.x * DB;
+  const int nd = min(DB, g.nf - d0);
+  const z64* Sb = S + ((long)b * g.nf + d0) * N4;
+  for (int e = t; e < nd * N4; e += 256) sS[e] = Sb[e];
+  __syncthreads();
+  const z64* C = ch == 0 ? c_d : c_m;
+  const int dl = t % DB, rl = t / DB;
+  if (dl >= nd) return;
+  const int d = d0 + dl;
+  const z64* sd = sS + dl * N4;
+  for (int r = rl; r < N; r += 256 / DB) {
+    z64 cc[N2];
+    const z64* crow = C + (long)r * N + d * N2;
+#pragma unroll
+    for (int i = 0; i < N2; ++i) cc[i] = crow[i];
+    cplx<T>* mrow = Mb + (long)r * ldm + d * N2;
+#pragma unroll
+    for (int sq = 0; sq < N2; ++sq) {
       z64 acc = mk(0.0, 0.0);
-      for (int d = 0; d < g.nf; ++d) acc = cadd(acc, rrow[d * n2 + t]);
-      gamq[(long)b * n2 * N + t + (long)r * n2] = acc;
+#pragma unroll
+      for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
+      mrow[sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
     }
   }
 }
 
 template <typename T>
-void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
-                  const z64* opgm, const z64* B, cplx<T>* M, z64* gamq, hipStream_t s) {
-  dim3 grid(g.N, 2, npts);
-  size_t lds = (size_t)g.N * sizeof(z64);
-  hipLaunchKernelGGL(k_build<T>, grid, dim3(256), lds, s, g, ldm, S, cd, cm, opgd, opgm, B, M, gamq);
+void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* B,
+                  cplx<T>* M, hipStream_t s) {
+#define ADGA_BUILD_CASE(NN)                                                                            \
+  case NN: {                                                                                           \
+    constexpr int DB = NN <= 3 ? 8 : 2;                                                                \
+    dim3 grid((g.nf + DB - 1) / DB + 1, 2, npts);                                                     \
+    hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, ldm, S, cd, cm, B, M);             \
+    break;                                                                                             \
+  }
+  switch (g.n) {
+    ADGA_BUILD_CASE(1)
+    ADGA_BUILD_CASE(2)
+    ADGA_BUILD_CASE(3)
+    ADGA_BUILD_CASE(4)
+    ADGA_BUILD_CASE(5)
+    default: break;
+  }
+#undef ADGA_BUILD_CASE
 }
-template void launch_build<float>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, const z64*,
-                                  const z64*, c32*, z64*, hipStream_t);
-template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, const z64*,
-                                   const z64*, z64*, z64*, hipStream_t);
+template void launch_build<float>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, c32*,
+                                  hipStream_t);
+template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, z64*,
+                                   hipStream_t);
 
 // ---------------------------------------------------------------- A7/A8 refinement residuals
 // Z[m][d*n2 + i'][s] = sum_c' S_b[d][c'][i'] V[m][d*n2 + c'][s]   (m = 2b + ch), i.e. Z = blockdiag(S)^T V,
@@ -260,6 +283,29 @@ __global__ void __launch_bounds__(256) k_resid_norm(Geom g, const z64* __restric
     __syncthreads();
   }
   if (t == 0) rel[m] = sb[0] > 0.0 ? sr[0] / sb[0] : (sr[0] > 0.0 ? INFINITY : 0.0);
+}
+
+// Zg[2b][d*n2 + i'][s] = S_b[d][s][i'], so that C^T Zg (k_resid_gemm) is
+// gamma^q_d(s, r) = sum_d sum_i' S_b[d][s][i'] C_d(d*n2 + i', r) = sum_nu chi0^{nl,q} F^w_d
+// (main.F90:693-700, the accumulation of rectanglework over nu).
+__global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __restrict__ Z) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)npts * g.N * g.n2;
+  if (idx >= tot) return;
+  int s = (int)(idx % g.n2);
+  long bx = idx / g.n2;
+  int x = (int)(bx % g.N);
+  int b = (int)(bx / g.N);
+  int d = x / g.n2, ip = x % g.n2;
+  Z[((long)(2 * b) * g.N + x) * g.n2 + s] = S[((long)b * g.nf + d) * g.n4 + s * g.n2 + ip];
+}
+
+// gamma^q_d for every point of the batch, into G2 with the residual layout (point b at 2*b).
+void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
+  long tot = (long)npts * g.N * g.n2;
+  hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
+  dim3 grid((g.N + 63) / 64, (npts * g.n2 + 63) / 64);
+  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 0, cd, nullptr, nullptr, Z, G2);
 }
 
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.

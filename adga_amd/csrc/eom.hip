@@ -62,7 +62,7 @@ __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* _
   long x = e / g.n2;
   const z64* ed = eta + (long)(2 * b) * per;
   const z64* em = eta + (long)(2 * b + 1) * per;
-  const z64* gq = gamq + (long)b * per;
+  const z64* gq = gamq + (long)(2 * b) * per;  // residual layout: point b at slot 2b
   z64 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0);
   for (int a = 0; a < g.n2; ++a) {
     z64 uu = u[s + a * g.n2], uw = ut[s + a * g.n2];
@@ -169,10 +169,14 @@ void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, cons
                      eta, bubloc, chi_out, bub_out);
 }
 
-void launch_eom(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* u, const z64* ut,
-                const z64* eta, const z64* gamq, z64* mt, const z64* gt, z64* sigma, hipStream_t s) {
+void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
+                 hipStream_t s) {
   long tot = (long)npts * g.n2 * g.N;
   hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt);
+}
+
+void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,
+                       const z64* gt, z64* sigma, hipStream_t s) {
   long te = (long)g.n2 * g.iwf * g.nkp_eom;
   hipLaunchKernelGGL(k_eom, dim3((unsigned)((te + 255) / 256)), dim3(256), 0, s, g, iwb, npts, qidx, keom, mt, gt,
                      sigma);

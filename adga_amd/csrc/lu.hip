@@ -50,7 +50,7 @@ struct Mfma<double> {
 
 constexpr int NT = 256;           // threads per workgroup (4 waves)
 constexpr int TM = 64, TN = 64;   // workgroup GEMM tile (complex elements); 32x32 per wave
-constexpr int KC = 16;            // K chunk staged through LDS
+constexpr int KC = 32;            // K chunk staged through LDS
 
 enum { OP_LEAF = 0, OP_TRSM_L = 1, OP_TRSM_U = 2, OP_GEMM = 3, OP_SWAP_RHS = 4 };
 
@@ -65,11 +65,15 @@ struct LuLds {
 };
 
 // ---------------------------------------------------------------- GEMM on MFMA
+// C[r0:r1, c0:c1] -= A[r0:r1, k0:k1] B[k0:k1, c0:c1] over 64x64 output tiles (32x32 per wave).
+// K is staged through LDS in chunks of KC; the next chunk's global loads are issued into
+// registers before the current chunk's MFMAs, so load latency overlaps the matrix work.
 template <typename T, int LW>
 __device__ __forceinline__ void wg_gemm(cplx<T>* __restrict__ M, int ldm, int r0, int r1, int c0, int c1, int k0,
                                         int k1, LuLds<T, LW>& L) {
   using V = typename Mfma<T>::V;
   using C = cplx<T>;
+  constexpr int PA = TM * KC / NT, PB = KC * TN / NT;  // prefetched elements per thread
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lg = lane >> 4, part = lg & 1, kcl = lg >> 1;
@@ -87,20 +91,46 @@ __device__ __forceinline__ void wg_gemm(cplx<T>* __restrict__ M, int ldm, int r0
             accr[rb][cb][v] = c.re;
             acci[rb][cb][v] = c.im;
           }
+      C pa[PA], pb[PB];
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        int e = t + i * NT, row = rt + e / KC, k = k0 + e % KC;
+        pa[i] = (row < r1 && k < k1) ? M[(long)row * ldm + k] : mk<T>(0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        int e = t + i * NT, k = k0 + e / TN, col = ct + e % TN;
+        pb[i] = (k < k1 && col < c1) ? M[(long)k * ldm + col] : mk<T>(0, 0);
+      }
       for (int kb = k0; kb < k1; kb += KC) {
-        for (int e = t; e < TM * KC; e += NT) {
-          int rr = e / KC, kk = e % KC;
-          int row = rt + rr, k = kb + kk;
-          L.At[kk][rr] = (row < r1 && k < k1) ? M[(long)row * ldm + k] : mk<T>(0, 0);
+        __syncthreads();  // the previous chunk's MFMAs are done with LDS
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+          int e = t + i * NT;
+          L.At[e % KC][e / KC] = pa[i];
         }
-        for (int e = t; e < KC * TN; e += NT) {
-          int kk = e / TN, cc = e % TN;
-          int k = kb + kk, col = ct + cc;
-          L.Bs[kk][cc] = (k < k1 && col < c1) ? M[(long)k * ldm + col] : mk<T>(0, 0);
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+          int e = t + i * NT;
+          L.Bs[e / TN][e % TN] = pb[i];
         }
         __syncthreads();
+        const int kn = kb + KC;
+        if (kn < k1) {
 #pragma unroll
-        for (int kk = 0; kk < KC; kk += 2) {
+          for (int i = 0; i < PA; ++i) {
+            int e = t + i * NT, row = rt + e / KC, k = kn + e % KC;
+            pa[i] = (row < r1 && k < k1) ? M[(long)row * ldm + k] : mk<T>(0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < PB; ++i) {
+            int e = t + i * NT, k = kn + e / TN, col = ct + e % TN;
+            pb[i] = (k < k1 && col < c1) ? M[(long)k * ldm + col] : mk<T>(0, 0);
+          }
+        }
+        const int kend = min(KC, k1 - kb);
+#pragma unroll 4
+        for (int kk = 0; kk < kend; kk += 2) {
           T ar[2], ai[2], bb[2];
 #pragma unroll
           for (int rb = 0; rb < 2; ++rb) {
@@ -121,7 +151,6 @@ __device__ __forceinline__ void wg_gemm(cplx<T>* __restrict__ M, int ldm, int r0
               acci[rb][cb] = Mfma<T>::op(ai[rb], bb[cb], acci[rb][cb]);
             }
         }
-        __syncthreads();
       }
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
@@ -132,6 +161,7 @@ __device__ __forceinline__ void wg_gemm(cplx<T>* __restrict__ M, int ldm, int r0
             int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
             if (row < r1 && col < c1) M[(long)row * ldm + col] = mk<T>(accr[rb][cb][v], acci[rb][cb][v]);
           }
+      __syncthreads();  // LDS reuse by the next tile
     }
   }
 }

@@ -66,7 +66,11 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
 
 /*
  * Options: "batch" (points per device batch), "refine_tol" (relative residual max|r|/max|b|),
- * "refine_maxit".  Returns ADGA_E_ARG for an unknown key or value.
+ * "refine_maxit", "bubble" (0: direct k sum per point; 1: all q of the current omega at once by
+ * k-grid DFT, cached per omega; 2: DFT when a batch has >= 64 points or the cache is valid, the
+ * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
+ * G by k-grid DFT when omega changes or the self-energy is read, the default).
+ * Returns ADGA_E_ARG for an unknown key or value.
  */
 int adga_set_option(adga_ctx *ctx, const char *key, double value);
 

@@ -34,17 +34,21 @@ def _compare_golden(out, gold, wl, tol):
         assert rel_err(out["bubble_loc"][w], gold["bubble_loc"][:, :, i]) < tol
 
 
+MODES = [(1, 1), (0, 0), (1, 0), (0, 1)]  # (bubble, eom): 1 = k-grid DFT, 0 = direct sums
+
+
+@pytest.mark.parametrize("modes", MODES, ids=lambda m: f"bubble{m[0]}-eom{m[1]}")
 @pytest.mark.parametrize("precision", ["fp64", "refined"])
 @pytest.mark.parametrize("name", ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8"])
-def test_gpu_matches_reference_golden(name, precision):
+def test_gpu_matches_reference_golden(name, precision, modes):
     from golden.make_golden import FIXTURES
     api = _api()
     gold = load_golden(name)
     wl = synth.workload(FIXTURES[name][0])
     inp = synth.build_inputs(wl)
     pts = [tuple(p) for p in gold["points"].tolist()]
-    print(f"case {name} {precision} points={len(pts)}", flush=True)
-    with api.Context(inp, batch=512) as ctx:
+    print(f"case {name} {precision} modes={modes} points={len(pts)}", flush=True)
+    with api.Context(inp, batch=512, bubble=modes[0], eom=modes[1]) as ctx:
         out = api.run_loop(ctx, pts, lambda w, c: synth.chi_loc(inp, w, c), precision)
         st = ctx.stats()
     assert (out["status"] == 0).all()
@@ -114,9 +118,9 @@ def test_gpu_full_size_against_oracle(t2g_full):
     pts = [(0, 0), (0, 1234), (3, 777)]
     cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
     ref = hp.run(pts, cl)
-    for precision in ("fp64", "refined"):
-        print(f"full size {precision}", flush=True)
-        with api.Context(inp) as ctx:
+    for precision, bubble in (("fp64", 0), ("refined", 0), ("refined", 1)):
+        print(f"full size {precision} bubble={bubble}", flush=True)
+        with api.Context(inp, bubble=bubble) as ctx:
             out = api.run_loop(ctx, pts, cl, precision)
         for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
             e = rel_err(out[k], ref[k])
