@@ -337,7 +337,7 @@ __device__ __forceinline__ void wg_trsm_u(cplx<T>* __restrict__ M, int ldm, int 
 
 // ---------------------------------------------------------------- the kernel
 template <typename T, int RPT, int LW>
-__global__ void __launch_bounds__(NT) k_lu(int N, int W, int ldm, int nops, const int* __restrict__ ops,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_lu(int N, int W, int ldm, int nops, const int* __restrict__ ops,
                                            cplx<T>* __restrict__ Mall, int* __restrict__ pivall,
                                            int* __restrict__ info) {
   __shared__ LuLds<T, LW> L;
