@@ -16,13 +16,12 @@ template <typename T>
 void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* B,
                   cplx<T>* M, hipStream_t s);
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s);
+size_t lu_inv_elems(int N);
+std::vector<int> lu_ops_factor(int N, int W);
+std::vector<int> lu_ops_solve(int N, int W);
 template <typename T>
-int lu_leaf_width(int N);
-std::vector<int> lu_ops_factor(int N, int W, int lw);
-std::vector<int> lu_ops_solve(int N, int W, int lw);
-template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, int nops, cplx<T>* M, int* piv, int* info,
-               hipStream_t s);
+bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, int nops, cplx<T>* M, int* piv, cplx<T>* inv,
+               int* info, hipStream_t s);
 template <typename T>
 void launch_rhs_io(int nmat, int N, int nrhs, int ldm, int stage, cplx<T>* M, z64* Y, const z64* R, hipStream_t s);
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,

@@ -40,6 +40,7 @@ struct adga_ctx {
   z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
       *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
   void* M = nullptr;
+  void* inv = nullptr;  // per-matrix panel inverses of the LU
   double* rel = nullptr;
   int *piv = nullptr, *info = nullptr, *qidx = nullptr;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
@@ -102,7 +103,7 @@ void upload(z64* d, const double* h, size_t count, hipStream_t s) {
 
 void free_batch(adga_ctx* c) {
   dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
-  dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->rel);
+  dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->inv); dfree(c->rel);
   dfree(c->piv); dfree(c->info); dfree(c->qidx);
   c->cap = 0;
 }
@@ -125,6 +126,7 @@ void ensure_batch(adga_ctx* c, int nb) {
   dalloc(&c->chi_out, 2 * nb * (size_t)g.n4);
   dalloc(&c->bub_out, nb * (size_t)g.n4);
   dalloc((char**)&c->M, 2 * nb * (size_t)g.N * c->ldm * sizeof(z64));
+  dalloc((char**)&c->inv, 2 * nb * adga::lu_inv_elems(g.N) * sizeof(z64));
   dalloc(&c->rel, 2 * nb);
   dalloc(&c->piv, 2 * nb * (size_t)g.N);
   dalloc(&c->info, 2 * nb);
@@ -178,9 +180,9 @@ int* upload_ops(const std::vector<int>& v) {
 // prec 0: complex128 LU, prec 1: complex64 LU
 void ensure_ops(adga_ctx* c, int prec) {
   if (c->ops_f[prec]) return;
+  (void)prec;
   const int N = c->g.N, W = c->g.N + c->g.n2;
-  const int lw = prec == 0 ? adga::lu_leaf_width<double>(N) : adga::lu_leaf_width<float>(N);
-  std::vector<int> f = adga::lu_ops_factor(N, W, lw), so = adga::lu_ops_solve(N, W, lw);
+  std::vector<int> f = adga::lu_ops_factor(N, W), so = adga::lu_ops_solve(N, W);
   c->ops_f[prec] = upload_ops(f);
   c->ops_s[prec] = upload_ops(so);
   c->nops_f[prec] = (int)(f.size() / 8);
@@ -394,7 +396,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, c->ops_f[0], c->nops_f[0], (z64*)c->M, c->piv,
-                                        c->info, s);
+                                        (z64*)c->inv, c->info, s);
         CK(hipEventRecord(lu1, s));
         if (lu_ok) {
           adga::launch_rhs_io<double>(nmat, g.N, n2, c->ldm, 0, (z64*)c->M, c->Y, nullptr, s);
@@ -410,8 +412,8 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->Bp, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
-        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_f[1], c->nops_f[1], (c32*)c->M, c->piv, c->info,
-                                       s);
+        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_f[1], c->nops_f[1], (c32*)c->M, c->piv,
+                                       (c32*)c->inv, c->info, s);
         CK(hipEventRecord(lu1, s));
         if (lu_ok) adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 0, (c32*)c->M, c->Y, nullptr, s);
         relh.resize(nmat);
@@ -429,7 +431,8 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
           // correction: the residual (rounded) replaces the right-hand-side columns, solve with the
           // complex64 factors, add the correction to the FP64 solution
           adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 2, (c32*)c->M, nullptr, c->R, s);
-          adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_s[1], c->nops_s[1], (c32*)c->M, c->piv, nullptr, s);
+          adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_s[1], c->nops_s[1], (c32*)c->M, c->piv, (c32*)c->inv,
+                                 nullptr, s);
           adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 1, (c32*)c->M, c->Y, nullptr, s);
           iters = it + 1;
         }
@@ -580,9 +583,11 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dalloc(&dpiv, (size_t)n);
     dalloc(&dinfo, 1);
     CK(hipMemcpy(dM, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
-    std::vector<int> ops = adga::lu_ops_factor(n, W, adga::lu_leaf_width<double>(n));
+    std::vector<int> ops = adga::lu_ops_factor(n, W);
     int* dops = upload_ops(ops);
-    bool ok = adga::launch_lu<double>(1, n, W, ldm, dops, (int)(ops.size() / 8), dM, dpiv, dinfo, nullptr);
+    z64* dinv = nullptr;
+    dalloc(&dinv, adga::lu_inv_elems(n));
+    bool ok = adga::launch_lu<double>(1, n, W, ldm, dops, (int)(ops.size() / 8), dM, dpiv, dinv, dinfo, nullptr);
     if (ok) {
       CK(hipGetLastError());
       CK(hipMemcpy(h.data(), dM, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
@@ -592,6 +597,7 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dfree(dpiv);
     dfree(dinfo);
     dfree(dops);
+    dfree(dinv);
     if (!ok) return ADGA_E_ARG;
     if (*ierr != 0) return ADGA_E_SINGULAR;
     z64* dst = reinterpret_cast<z64*>(m);
