@@ -137,10 +137,11 @@ class Context:
         self._check(lib.adga_reset_sigma(self.h))
 
     def stats(self):
-        out = np.zeros(11)
-        self._check(lib.adga_get_stats(self.h, _dptr(out), 11))
+        out = np.zeros(14)
+        self._check(lib.adga_get_stats(self.h, _dptr(out), 14))
         keys = ["refine_iters", "refine_resid", "t_local", "t_chi0", "t_gammas", "t_inversion", "t_eta_rest",
-                "t_eom", "t_chi", "t_lu_kernel", "n_lu_launches"]
+                "t_eom", "t_chi", "t_lu_kernel", "n_lu_launches", "t_gemm_dominant", "flops_gemm_dominant",
+                "n_gemm_dominant"]
         return dict(zip(keys, out.tolist()))
 
 

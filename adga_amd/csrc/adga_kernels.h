@@ -13,17 +13,29 @@ void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z
 void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s);
 void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64* S, hipStream_t s);
 template <typename T>
-void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* B,
-                  cplx<T>* M, hipStream_t s);
+void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, cplx<T>* M,
+                  hipStream_t s);
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s);
 size_t lu_inv_elems(int N);
 std::vector<int> lu_ops_factor(int N, int W);
-std::vector<int> lu_ops_solve(int N, int W);
+// Events around the launches of the largest GEMM tile shape (the factorisation's dominant kernel):
+// mark() records one event; collect() (after the stream has completed) returns the summed
+// milliseconds of the bracketed launches and resets.
+struct LuTimer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  double flops = 0.0;  // algorithmic flops of the bracketed launches (8 m n K each)
+  int launches = 0;
+  void mark(hipStream_t s);
+  double collect();
+  ~LuTimer();
+};
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, int nops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s);
+bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
+               int* info, hipStream_t s, LuTimer* timer = nullptr);
 template <typename T>
-void launch_rhs_io(int nmat, int N, int nrhs, int ldm, int stage, cplx<T>* M, z64* Y, const z64* R, hipStream_t s);
+bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s);
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
                 const z64* opgm, z64* Z, z64* B, hipStream_t s);
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,

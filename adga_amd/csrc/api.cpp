@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -44,9 +46,7 @@ struct adga_ctx {
   double* rel = nullptr;
   int *piv = nullptr, *info = nullptr, *qidx = nullptr;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
-  int* ops_f[2] = {nullptr, nullptr};
-  int* ops_s[2] = {nullptr, nullptr};
-  int nops_f[2] = {0, 0}, nops_s[2] = {0, 0};
+  std::vector<int> ops_f[2];  // LU operation lists (host) per precision
   // accumulated self-energies
   z64 *sig_nl = nullptr, *sig_dmft = nullptr;
   // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
@@ -57,7 +57,8 @@ struct adga_ctx {
   int bub_iwb = 0;
   bool eom_pending = false;
   int eom_iwb = 0;
-  double stats[11] = {0};  // see adga_get_stats
+  double stats[14] = {0};  // see adga_get_stats
+  adga::LuTimer lut;       // events around the dominant GEMM kernel's launches
 };
 
 namespace {
@@ -82,6 +83,12 @@ int guarded(adga_ctx* c, F&& f) {
     snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)he.e, hipGetErrorString(he.e), he.what);
     if (c) c->err = buf;
     return he.e == hipErrorOutOfMemory ? ADGA_E_NOMEM : ADGA_E_HIP;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return ADGA_E_NOMEM;
+  } catch (const std::exception& ex) {
+    if (c) c->err = std::string("host exception: ") + ex.what();
+    return ADGA_E_HIP;
   } catch (...) {
     if (c) c->err = "unexpected host exception";
     return ADGA_E_HIP;
@@ -170,23 +177,12 @@ void eom_flush(adga_ctx* c) {
   c->eom_pending = false;
 }
 
-int* upload_ops(const std::vector<int>& v) {
-  int* d = nullptr;
-  dalloc(&d, v.size());
-  CK(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
-  return d;
-}
-
 // prec 0: complex128 LU, prec 1: complex64 LU
 void ensure_ops(adga_ctx* c, int prec) {
-  if (c->ops_f[prec]) return;
+  if (!c->ops_f[prec].empty()) return;
   (void)prec;
-  const int N = c->g.N, W = c->g.N + c->g.n2;
-  std::vector<int> f = adga::lu_ops_factor(N, W), so = adga::lu_ops_solve(N, W);
-  c->ops_f[prec] = upload_ops(f);
-  c->ops_s[prec] = upload_ops(so);
-  c->nops_f[prec] = (int)(f.size() / 8);
-  c->nops_s[prec] = (int)(so.size() / 8);
+  const int N = c->g.N;
+  c->ops_f[prec] = adga::lu_ops_factor(N, N);
 }
 
 }  // namespace
@@ -223,8 +219,7 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     g.nkp_eom = d->nkp_eom;
     g.beta = d->beta;
     g.mu = d->mu;
-    int W = g.N + g.n2;
-    c->ldm = (W + 15) / 16 * 16;
+    c->ldm = (g.N + 15) / 16 * 16;
     c->q0.resize(g.nqp);
     for (int i = 0; i < g.nqp; ++i) {
       if (q_data[i] < 1 || q_data[i] > g.nkp) throw HipError{hipErrorInvalidValue, "q_data out of range"};
@@ -342,8 +337,10 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     hipStream_t s = c->stream;
-    const int n2 = g.n2, W = g.N + g.n2;
-    std::fill(c->stats, c->stats + 11, 0.0);
+    const int n2 = g.n2;
+    std::fill(c->stats, c->stats + 14, 0.0);
+    c->lut.used = 0;
+    c->lut.flops = 0.0;
     int worst = ADGA_OK;
     hipEvent_t ev[8], lu0, lu1;
     CK(hipEventCreate(&lu0));
@@ -392,14 +389,16 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       bool lu_ok;
       ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
       if (precision == ADGA_PREC_FP64) {
-        adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, c->Bp, (z64*)c->M, s);
+        adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
-        lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, c->ops_f[0], c->nops_f[0], (z64*)c->M, c->piv,
-                                        (z64*)c->inv, c->info, s);
+        lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, c->ops_f[0], (z64*)c->M, c->piv,
+                                        (z64*)c->inv, c->info, s, &c->lut);
         CK(hipEventRecord(lu1, s));
+        if (lu_ok)
+          lu_ok = adga::launch_lu_solve<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->piv, (const z64*)c->inv,
+                                                c->Bp, c->Y, false, s);
         if (lu_ok) {
-          adga::launch_rhs_io<double>(nmat, g.N, n2, c->ldm, 0, (z64*)c->M, c->Y, nullptr, s);
           // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
           relh.resize(nmat);
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
@@ -409,13 +408,15 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
             if (std::isfinite(r)) maxrel = std::max(maxrel, r);
         }
       } else {
-        adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, c->Bp, (c32*)c->M, s);
+        adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
-        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_f[1], c->nops_f[1], (c32*)c->M, c->piv,
-                                       (c32*)c->inv, c->info, s);
+        lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, c->ops_f[1], (c32*)c->M, c->piv,
+                                       (c32*)c->inv, c->info, s, &c->lut);
         CK(hipEventRecord(lu1, s));
-        if (lu_ok) adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 0, (c32*)c->M, c->Y, nullptr, s);
+        if (lu_ok)
+          lu_ok = adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv,
+                                               c->Bp, c->Y, false, s);
         relh.resize(nmat);
         for (int it = 0; lu_ok; ++it) {
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
@@ -428,12 +429,10 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
             else maxrel = std::max(maxrel, r);
           }
           if (bad || maxrel <= c->refine_tol || it >= c->refine_maxit) break;
-          // correction: the residual (rounded) replaces the right-hand-side columns, solve with the
-          // complex64 factors, add the correction to the FP64 solution
-          adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 2, (c32*)c->M, nullptr, c->R, s);
-          adga::launch_lu<float>(nmat, g.N, W, c->ldm, c->ops_s[1], c->nops_s[1], (c32*)c->M, c->piv, (c32*)c->inv,
-                                 nullptr, s);
-          adga::launch_rhs_io<float>(nmat, g.N, n2, c->ldm, 1, (c32*)c->M, c->Y, nullptr, s);
+          // correction: solve with the complex64 factors for the (rounded) residual and add the
+          // correction to the FP64 solution
+          adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv, c->R,
+                                       c->Y, true, s);
           iters = it + 1;
         }
       }
@@ -480,6 +479,10 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       CK(hipEventElapsedTime(&lums, lu0, lu1));
       c->stats[9] += lums;                 // the batched LU kernel alone
       c->stats[10] += 1;                   // its launches
+      c->stats[11] += c->lut.collect();    // the dominant GEMM kernel: milliseconds,
+      c->stats[12] += c->lut.flops;        // algorithmic flops,
+      c->stats[13] += c->lut.launches;     // launches
+      c->lut.flops = 0.0;
       c->stats[0] = std::max(c->stats[0], (double)iters);
       c->stats[1] = std::max(c->stats[1], maxrel);
       for (int b = 0; b < nb; ++b) {
@@ -539,7 +542,7 @@ int adga_reset_sigma(adga_ctx* c) {
 
 int adga_get_stats(adga_ctx* c, double* out, int nout) {
   if (!c || !out || nout < 0) return ADGA_E_ARG;
-  for (int i = 0; i < nout && i < 11; ++i) out[i] = c->stats[i];
+  for (int i = 0; i < nout && i < 14; ++i) out[i] = c->stats[i];
   return ADGA_OK;
 }
 
@@ -557,7 +560,6 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->xd); dfree(c->xm); dfree(c->cd); dfree(c->cm); dfree(c->gamd); dfree(c->gamm); dfree(c->opgd);
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
   dfree(c->sig_nl); dfree(c->sig_dmft);
-  for (int p = 0; p < 2; ++p) { dfree(c->ops_f[p]); dfree(c->ops_s[p]); }
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -584,10 +586,9 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dalloc(&dinfo, 1);
     CK(hipMemcpy(dM, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
     std::vector<int> ops = adga::lu_ops_factor(n, W);
-    int* dops = upload_ops(ops);
     z64* dinv = nullptr;
     dalloc(&dinv, adga::lu_inv_elems(n));
-    bool ok = adga::launch_lu<double>(1, n, W, ldm, dops, (int)(ops.size() / 8), dM, dpiv, dinv, dinfo, nullptr);
+    bool ok = adga::launch_lu<double>(1, n, W, ldm, ops, dM, dpiv, dinv, dinfo, nullptr);
     if (ok) {
       CK(hipGetLastError());
       CK(hipMemcpy(h.data(), dM, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
@@ -596,7 +597,6 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
     dfree(dM);
     dfree(dpiv);
     dfree(dinfo);
-    dfree(dops);
     dfree(dinv);
     if (!ok) return ADGA_E_ARG;
     if (*ierr != 0) return ADGA_E_SINGULAR;

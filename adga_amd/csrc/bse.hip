@@ -12,6 +12,7 @@
 // from the unrounded S and C, so the residual is on the scale of eta itself.
 // M is stored row-major per matrix (row r, column c at r*ldm + c).
 #include "adga_kernels.h"
+#include "mfma_gemm.h"
 
 namespace adga {
 
@@ -96,30 +97,19 @@ void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64
 
 // ---------------------------------------------------------------- A6: BSE build
 // M_{b,ch}(r, d*n2 + s) = delta - R_d(s, r),  R_d(s, r) = sum_i' S_b[d][s][i'] C_ch(d*n2 + i', r)
-// (main.F90:693-700, transposed: M = A^T); M(r, N + s) = B(r, s), the right-hand side of
-// M eta^T = B (launch_rhs); padding columns zero.  A workgroup owns DB frequency blocks d of one
+// (main.F90:693-700, transposed: M = A^T).  A workgroup owns DB frequency blocks d of one
 // (point, channel) for all rows: S_d sits in LDS, and each thread turns the n2 contiguous values
-// C(d*n2 + ., r) (a row slice of C^T) into n2 outputs.  The extra last workgroup along x writes
-// the right-hand-side and padding columns.
+// C(d*n2 + ., r) (a row slice of C^T) into n2 outputs.  Padding columns [N, ldm) are not written
+// (never read).
 template <typename T, int NN>
 __global__ void __launch_bounds__(256) k_build(Geom g, int ldm, const z64* __restrict__ S, const z64* __restrict__ c_d,
-                                               const z64* __restrict__ c_m, const z64* __restrict__ B,
-                                               cplx<T>* __restrict__ M) {
+                                               const z64* __restrict__ c_m, cplx<T>* __restrict__ M) {
   constexpr int N2 = NN * NN, N4 = N2 * N2;
   constexpr int DB = NN <= 3 ? 8 : 2;
   __shared__ z64 sS[DB * N4];
   const int ch = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
   const int N = g.N;
   cplx<T>* Mb = M + (long)(2 * b + ch) * N * ldm;
-  if ((int)blockIdx.x == (int)gridDim.x - 1) {  // right-hand side + padding columns
-    const int wcol = ldm - N;
-    const z64* Bm = B + (long)(2 * b + ch) * N * N2;
-    for (long e = t; e < (long)N * wcol; e += 256) {
-      int r = (int)(e / wcol), c = (int)(e % wcol);
-      Mb[(long)r * ldm + N + c] = c < N2 ? ccast<T>(Bm[(long)r * N2 + c]) : mk<T>(0, 0);
-    }
-    return;
-  }
   const int d0 = blockIdx.x * DB;
   const int nd = min(DB, g.nf - d0);
   const z64* Sb = S + ((long)b * g.nf + d0) * N4;
@@ -136,7 +126,8 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int ldm, const z64* __res
 #pragma unroll
     for (int i = 0; i < N2; ++i) cc[i] = crow[i];
     cplx<T>* mrow = Mb + (long)r * ldm + d * N2;
-#pragma unroll
+    // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
+#pragma unroll 1
     for (int sq = 0; sq < N2; ++sq) {
       z64 acc = mk(0.0, 0.0);
 #pragma unroll
@@ -147,13 +138,13 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int ldm, const z64* __res
 }
 
 template <typename T>
-void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, const z64* B,
-                  cplx<T>* M, hipStream_t s) {
+void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, cplx<T>* M,
+                  hipStream_t s) {
 #define ADGA_BUILD_CASE(NN)                                                                            \
   case NN: {                                                                                           \
     constexpr int DB = NN <= 3 ? 8 : 2;                                                                \
-    dim3 grid((g.nf + DB - 1) / DB + 1, 2, npts);                                                     \
-    hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, ldm, S, cd, cm, B, M);             \
+    dim3 grid((g.nf + DB - 1) / DB, 2, npts);                                                         \
+    hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, ldm, S, cd, cm, M);                \
     break;                                                                                             \
   }
   switch (g.n) {
@@ -166,10 +157,8 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
   }
 #undef ADGA_BUILD_CASE
 }
-template void launch_build<float>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, c32*,
-                                  hipStream_t);
-template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, const z64*, z64*,
-                                   hipStream_t);
+template void launch_build<float>(const Geom&, int, int, const z64*, const z64*, const z64*, c32*, hipStream_t);
+template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, z64*, hipStream_t);
 
 // ---------------------------------------------------------------- A7/A8 refinement residuals
 // Z[m][d*n2 + i'][s] = sum_c' S_b[d][c'][i'] V[m][d*n2 + c'][s]   (m = 2b + ch), i.e. Z = blockdiag(S)^T V,
@@ -197,68 +186,40 @@ __global__ void k_resid_z(Geom g, int nmat, const z64* __restrict__ S, const z64
   Z[idx] = acc;
 }
 
-// R[m][r][s] = B[m][r][s] - Y[m][r][s] + sum_x C_ch(x, r) Z[m][x][s]  (B, Y may be null = 0);
-// one channel per launch.  GEMM over the shared C: rows r, columns (point, s); 64x64 tiles,
-// 16-deep K chunks.
-__global__ void __launch_bounds__(256) k_resid_gemm(Geom g, int npts, int ch, const z64* __restrict__ C,
-                                                    const z64* __restrict__ B, const z64* __restrict__ Y,
-                                                    const z64* __restrict__ Z, z64* __restrict__ R) {
-  constexpr int TS = 64, KC = 16;
-  __shared__ z64 tA[TS][KC + 1];
-  __shared__ z64 tB[KC][TS + 1];
-  const int N = g.N, n2 = g.n2;
-  const int ncol = npts * n2;
-  const int rt = blockIdx.x * TS, ct = blockIdx.y * TS;
-  const int t = threadIdx.x, ty = t / 16, tx = t % 16;
-  z64 acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = mk(0.0, 0.0);
-  for (int x0 = 0; x0 < N; x0 += KC) {
-    for (int e = t; e < TS * KC; e += 256) {
-      int rr = e / KC, kk = e % KC;
-      int r = rt + rr, x = x0 + kk;
-      tA[rr][kk] = (r < N && x < N) ? C[(long)x + (long)r * N] : mk(0.0, 0.0);  // C(x, r)
-    }
-    for (int e = t; e < TS * KC; e += 256) {
-      int kk = e / TS, cc = e % TS;
-      int col = ct + cc, x = x0 + kk;
-      z64 v = mk(0.0, 0.0);
-      if (col < ncol && x < N) {
-        int b = col / n2, s = col % n2;
-        v = Z[((long)(2 * b + ch) * N + x) * n2 + s];
-      }
-      tB[kk][cc] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KC; ++kk) {
-      z64 av[4], bv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = tA[ty * 4 + a][kk];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = tB[kk][tx * 4 + b];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = cfma(av[a], bv[b], acc[a][b]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      int r = rt + ty * 4 + a, col = ct + tx * 4 + b;
-      if (r < N && col < ncol) {
-        int pb = col / n2, s = col % n2;
-        long yi = ((long)(2 * pb + ch) * N + r) * n2 + s;
-        z64 v = B ? B[yi] : mk(0.0, 0.0);
-        if (Y) v = csub(v, Y[yi]);
-        R[yi] = cadd(v, acc[a][b]);
-      }
-    }
+// R_ch = B - Y + C_ch^T Z for every point of the batch (B, Y optional), on FP64 MFMA: one GEMM per
+// channel, (N x N) C_ch^T times the batch's N x (npts n2) right-hand sides, 128x64 tiles; Z, B, Y, R
+// in the per-matrix layout [(2b + ch) N + row] n2 + s.
+__global__ void __launch_bounds__(GEMM_NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* __restrict__ cm,
+             const z64* __restrict__ B, const z64* __restrict__ Y, const z64* __restrict__ Z, z64* __restrict__ R) {
+  constexpr int TMT = 128, TNT = 64;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[gemm_lds_bytes<double>()];
+  const int ch = ch0 + (int)blockIdx.y;
+  const z64* C = ch == 0 ? cd : cm;
+  const int N = g.N, n2 = g.n2, ncol = npts * n2;
+  const int ntc = (ncol + TNT - 1) / TNT;
+  const int rt = ((int)blockIdx.x / ntc) * TMT, ct = ((int)blockIdx.x % ntc) * TNT;
+  gemm_tile_acc<double, TMT, TNT, 2, 2>(
+      [=](int r, int x) { return C[(long)r * N + x]; },  // C^T(r, x) = C(x, r)
+      [=](int x, int col) {
+        const int b = col / n2, sq = col - b * n2;
+        return Z[((long)(2 * b + ch) * N + x) * n2 + sq];
+      },
+      [=](int, int) { return mk(0.0, 0.0); },
+      [=](int r, int col, z64 v) {
+        const int b = col / n2, sq = col - b * n2;
+        const long yi = ((long)(2 * b + ch) * N + r) * n2 + sq;
+        z64 o = B ? B[yi] : mk(0.0, 0.0);
+        if (Y) o = csub(o, Y[yi]);
+        R[yi] = cadd(o, v);
+      },
+      N, ncol, N, true, rt, ct, lds);
+}
+
+static void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* B,
+                       const z64* Y, const z64* Z, z64* R, hipStream_t s) {
+  const int ntr = (g.N + 127) / 128, ntc = (npts * g.n2 + 63) / 64;
+  hipLaunchKernelGGL(k_resid_gemm, dim3(ntr * ntc, nch), dim3(GEMM_NT), 0, s, g, npts, ch0, cd, cm, B, Y, Z, R);
 }
 
 // Per-matrix max |R| / max |B| (|.| = |re|+|im|), one workgroup per matrix; a zero right-hand
@@ -304,8 +265,7 @@ __global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __res
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
   long tot = (long)npts * g.N * g.n2;
   hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
-  dim3 grid((g.N + 63) / 64, (npts * g.n2 + 63) / 64);
-  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 0, cd, nullptr, nullptr, Z, G2);
+  resid_gemm(g, npts, 0, 1, cd, cd, nullptr, nullptr, Z, G2, s);
 }
 
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.
@@ -316,9 +276,7 @@ void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64*
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nmat, S, nullptr, opgd,
                      opgm, Z);
-  dim3 grid((g.N + 63) / 64, (npts * g.n2 + 63) / 64);
-  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 0, cd, nullptr, nullptr, Z, B);
-  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 1, cm, nullptr, nullptr, Z, B);
+  resid_gemm(g, npts, 0, 2, cd, cm, nullptr, nullptr, Z, B, s);
 }
 
 // R = B - M Y with M Y = Y - C^T (blockdiag(S)^T Y), in FP64 from the unrounded factors S and C.
@@ -328,9 +286,7 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nmat, S, Y, nullptr,
                      nullptr, Z);
-  dim3 grid((g.N + 63) / 64, (npts * g.n2 + 63) / 64);
-  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 0, cd, B, Y, Z, R);
-  hipLaunchKernelGGL(k_resid_gemm, grid, dim3(256), 0, s, g, npts, 1, cm, B, Y, Z, R);
+  resid_gemm(g, npts, 0, 2, cd, cm, B, Y, Z, R, s);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);
 }
 

@@ -1,10 +1,11 @@
-// A7/A8: batched complex LU with partial pivoting and the n^2 right-hand-side solves, one
-// workgroup per matrix, on gfx950 MFMA.
+// A7/A8: batched complex LU with partial pivoting, one workgroup per matrix, on gfx950 MFMA,
+// and the streaming solve of the n^2 right-hand sides against the factors.
 //
-// M is row-major (row r, column c at r*ldm + c) with the matrix in columns [0, N) and the
-// right-hand sides in [N, W).  The factorisation is recursive (left half, Schur update of the
-// right half, right half) down to panels of P columns, written out on the host as a list of
-// operations so the kernel is a flat loop:
+// M is row-major (row r, column c at r*ldm + c) with the matrix in columns [0, N); optional
+// right-hand sides riding along in columns [N, W) are transformed with the rows (the explicit
+// inverse uses that).  The factorisation is recursive (left half, Schur update of the right half,
+// right half) down to panels of P columns, written out on the host as a list of operations so the
+// kernel is a flat loop:
 //   PANEL(c0, c1)                    columns [c0, c1) (<= P) of rows [c0, N), factored left-looking
 //                                    leaf by leaf (LW columns in registers) with partial pivoting
 //                                    (|re|+|im| like izamax, first index on ties); each leaf's
@@ -15,65 +16,56 @@
 //                                    stored inverse of panel k0
 //   TRSM_U(k0, k1, c0, c1)           rows [k0,k1) of columns [c0,c1) <- U11^-1 (.)
 //   GEMM(r0, r1, c0, c1, k0, k1)     M[r0:r1, c0:c1] -= M[r0:r1, k0:k1] M[k0:k1, c0:c1]
-//   SWAP_RHS                         all interchanges, in order, on columns [N, W)
 // Large triangular solves are recursive too, so the work runs as GEMMs.  The GEMM runs on MFMA
 // (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64): a complex product is a real product with K
 // doubled and re/im interleaved along it, the lane's K slot selecting (re, -im) or (im, re) of the
 // A element and (re, im) of the B element.
+#include <algorithm>
 #include <vector>
 
 #include "adga_kernels.h"
+#include "mfma_gemm.h"
 
 namespace adga {
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int NT = 256;  // threads per workgroup (4 waves)
+constexpr int P = 32;    // panel width: the recursion's leaf, and the TRSM block
 
-template <typename T>
-struct Mfma;
-// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16] and B[l/16][l%16].
-// Accumulator: register v of lane l holds D[row(l/16, v)][l%16], with row = 4*(l/16) + v for the
-// f32 instruction and row = l/16 + 4*v for the f64 one (both measured on gfx950).
-template <>
-struct Mfma<float> {
-  typedef f32x4 V;
-  static __device__ __forceinline__ V op(float a, float b, V c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+enum { OP_PANEL = 0, OP_TRSM_L = 1, OP_TRSM_U = 2, OP_GEMM = 3 };
+
+// One GEMM or TRSM operation on every matrix of the batch, one tile per workgroup.
+//   mode 0: M[r0:r0+m, c0:c0+n] -= M[r0:r0+m, k0:k0+K] M[k0:k0+K, c0:c0+n]
+//   mode 1/2: M[k0:k0+K, c0:c0+n] <- D M[k0:k0+K, c0:c0+n] with D the stored inverse of the unit-lower
+//             (1) or upper (2) diagonal block of panel k0 (m = K <= P, in place)
+// Workgroups are dealt round-robin to the 8 XCDs; the decode keeps all tiles of one matrix on one
+// XCD and adjacent in time, so its A row panels and B column panels are shared through that L2.
+template <typename T, int TMT, int TNT, int WR, int WC>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_gemm(int nmat, int ntc, int ntile,
+                                                                                  long mstride, cplx<T>* __restrict__ Mall,
+                                             int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
+                                             const cplx<T>* __restrict__ invall, long istride) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[gemm_lds_bytes<T>()];
+  const int L = blockIdx.x, x = L & 7, sl = L >> 3;
+  const int mat = (sl / ntile) * 8 + x, tile = sl % ntile;
+  if (mat >= nmat) return;
+  const int rt = (tile / ntc) * TMT, ct = (tile % ntc) * TNT;
+  cplx<T>* M = Mall + (long)mat * mstride;
+  if (mode == 0) {
+    gemm_tile<T, TMT, TNT, WR, WC>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm,
+                                   M + (long)r0 * ldm + c0, ldm, m, n, K, false, rt, ct, lds);
+  } else {
+    const cplx<T>* D = invall + (long)mat * istride + (long)(k0 / P) * 2 * P * P + (mode == 2 ? P * P : 0);
+    cplx<T>* X = M + (long)k0 * ldm + c0;
+    gemm_tile<T, TMT, TNT, WR, WC>(D, P, X, ldm, X, ldm, K, n, K, true, rt, ct, lds);
   }
-  static __device__ __forceinline__ int row(int lg, int v) { return 4 * lg + v; }
-};
-template <>
-struct Mfma<double> {
-  typedef f64x4 V;
-  static __device__ __forceinline__ V op(double a, double b, V c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lg, int v) { return lg + 4 * v; }
-};
+}
 
-constexpr int NT = 256;           // threads per workgroup (4 waves)
-constexpr int TM = 64, TN = 64;   // workgroup GEMM tile (complex elements); 32x32 per wave
-constexpr int P = 32;             // panel width: the recursion's leaf, and the TRSM block
-static_assert(P <= TM, "a TRSM block is one row tile (the in-place solve relies on it)");
-
-// K chunk staged through LDS: 32 for complex64, 16 for complex128 (keeps two workgroups per CU)
-template <typename T>
-struct Kc {
-  static constexpr int value = sizeof(T) == 4 ? 32 : 16;
-};
-
-enum { OP_PANEL = 0, OP_TRSM_L = 1, OP_TRSM_U = 2, OP_GEMM = 3, OP_SWAP_RHS = 4 };
-
+// Panel LDS
 template <typename T, int LW>
-struct LuLds {
-  static constexpr int KC = Kc<T>::value;
-  union {
+struct PanelLds {
+  struct {
     struct {
-      cplx<T> At[KC][TM + 4];  // A chunk, transposed: At[k][row]
-      cplx<T> Bs[KC][TN + 4];  // B chunk: Bs[k][col]
-    } g;
-    struct {
-      cplx<T> Lb[P][P + 1];     // a panel's diagonal block
+      cplx<T> Lb[P][P + 1];     // the panel's diagonal block
       cplx<T> Xb[2][P][P + 1];  // its inverses being formed (L, U)
     } v;
   } u;
@@ -85,117 +77,11 @@ struct LuLds {
   int spiv[LW];
 };
 
-// ---------------------------------------------------------------- GEMM on MFMA
-// C[0:m, 0:n] (ldc) <- C - A B (assign = false) or A B (assign = true), A[0:m, 0:K] (lda),
-// B[0:K, 0:n] (ldb); 64x64 output tiles (32x32 per wave).  K is staged through LDS in chunks of
-// KC; the next chunk's global loads are issued into registers before the current chunk's MFMAs,
-// so load latency overlaps the matrix work.  A tile's output is written after all of its K has been
-// read by every wave, so C may alias B when one row tile spans all K rows of B (in-place solves).
-template <typename T, int LW>
-__device__ __forceinline__ void wg_gemm(const cplx<T>* A, int lda, const cplx<T>* B, int ldb, cplx<T>* Cm, int ldc,
-                                        int m, int n, int K, bool assign, LuLds<T, LW>& L) {
-  using V = typename Mfma<T>::V;
-  using C = cplx<T>;
-  constexpr int KC = Kc<T>::value;
-  constexpr int PA = TM * KC / NT, PB = KC * TN / NT;  // prefetched elements per thread
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int li = lane & 15, lg = lane >> 4, part = lg & 1, kcl = lg >> 1;
-  const T sgn = assign ? (T)1 : (T)-1;
-  for (int rt = 0; rt < m; rt += TM) {
-    for (int ct = 0; ct < n; ct += TN) {
-      V accr[2][2], acci[2][2];
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
-            C c = (!assign && row < m && col < n) ? Cm[(long)row * ldc + col] : mk<T>(0, 0);
-            accr[rb][cb][v] = c.re;
-            acci[rb][cb][v] = c.im;
-          }
-      C pa[PA], pb[PB];
-#pragma unroll
-      for (int i = 0; i < PA; ++i) {
-        int e = t + i * NT, row = rt + e / KC, k = e % KC;
-        pa[i] = (row < m && k < K) ? A[(long)row * lda + k] : mk<T>(0, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < PB; ++i) {
-        int e = t + i * NT, k = e / TN, col = ct + e % TN;
-        pb[i] = (k < K && col < n) ? B[(long)k * ldb + col] : mk<T>(0, 0);
-      }
-      for (int kb = 0; kb < K; kb += KC) {
-        __syncthreads();  // the previous chunk's MFMAs are done with LDS
-#pragma unroll
-        for (int i = 0; i < PA; ++i) {
-          int e = t + i * NT;
-          L.u.g.At[e % KC][e / KC] = pa[i];
-        }
-#pragma unroll
-        for (int i = 0; i < PB; ++i) {
-          int e = t + i * NT;
-          L.u.g.Bs[e / TN][e % TN] = pb[i];
-        }
-        __syncthreads();
-        const int kn = kb + KC;
-        if (kn < K) {
-#pragma unroll
-          for (int i = 0; i < PA; ++i) {
-            int e = t + i * NT, row = rt + e / KC, k = kn + e % KC;
-            pa[i] = (row < m && k < K) ? A[(long)row * lda + k] : mk<T>(0, 0);
-          }
-#pragma unroll
-          for (int i = 0; i < PB; ++i) {
-            int e = t + i * NT, k = kn + e / TN, col = ct + e % TN;
-            pb[i] = (k < K && col < n) ? B[(long)k * ldb + col] : mk<T>(0, 0);
-          }
-        }
-        const int kend = min(KC, K - kb);
-#pragma unroll 4
-        for (int kk = 0; kk < kend; kk += 2) {
-          T ar[2], ai[2], bb[2];
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb) {
-            C a = L.u.g.At[kk + kcl][wr + rb * 16 + li];
-            ar[rb] = sgn * (part ? -a.im : a.re);
-            ai[rb] = sgn * (part ? a.re : a.im);
-          }
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb) {
-            C b = L.u.g.Bs[kk + kcl][wc + cb * 16 + li];
-            bb[cb] = part ? b.im : b.re;
-          }
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-              accr[rb][cb] = Mfma<T>::op(ar[rb], bb[cb], accr[rb][cb]);
-              acci[rb][cb] = Mfma<T>::op(ai[rb], bb[cb], acci[rb][cb]);
-            }
-        }
-      }
-      __syncthreads();  // every wave has consumed its operands before a tile output (may alias B) lands
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
-            if (row < m && col < n) Cm[(long)row * ldc + col] = mk<T>(accr[rb][cb][v], acci[rb][cb][v]);
-          }
-    }
-  }
-}
-
 // ---------------------------------------------------------------- panel factorisation
 // Columns [c0, c1) of rows [c0, N); thread t holds rows c0 + t + i*NT (i < RPT) of the current leaf.
 template <typename T, int RPT, int LW>
 __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N, int W, int c0, int c1,
-                                         int* __restrict__ piv, cplx<T>* __restrict__ inv, LuLds<T, LW>& L,
+                                         int* __restrict__ piv, cplx<T>* __restrict__ inv, PanelLds<T, LW>& L,
                                          int& myinfo) {
   using C = cplx<T>;
   constexpr int NWAVE = NT / 64;
@@ -427,44 +313,190 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
   }
 }
 
-// ---------------------------------------------------------------- the kernel
+// ---------------------------------------------------------------- panel, pivots
+// PANEL(c0, c1) of every matrix of the batch, one workgroup per matrix.  info[m] keeps the first
+// zero pivot (1-based, LAPACK's info), or stays 0.
 template <typename T, int RPT, int LW>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_lu(int N, int W, int ldm, int nops, const int* __restrict__ ops, cplx<T>* __restrict__ Mall,
-     int* __restrict__ pivall, cplx<T>* __restrict__ invall, int* __restrict__ info) {
-  __shared__ LuLds<T, LW> L;
-  cplx<T>* M = Mall + (long)blockIdx.x * N * ldm;
-  int* piv = pivall + (long)blockIdx.x * N;
-  cplx<T>* inv = invall + (long)blockIdx.x * ((N + P - 1) / P) * 2 * P * P;
+__global__ void __launch_bounds__(NT) k_panel(int N, int W, int ldm, long mstride, int c0, int c1,
+                                              cplx<T>* __restrict__ Mall, int* __restrict__ pivall,
+                                              cplx<T>* __restrict__ invall, long istride, int* __restrict__ info) {
+  __shared__ PanelLds<T, LW> L;
+  const int mat = blockIdx.x;
   int myinfo = 0;
-  for (int o = 0; o < nops; ++o) {
-    const int* op = ops + 8 * o;
-    const int ty = op[0];
-    if (ty == OP_PANEL) {
-      wg_panel<T, RPT, LW>(M, ldm, N, W, op[1], op[2], piv, inv, L, myinfo);
-    } else if (ty == OP_TRSM_L || ty == OP_TRSM_U) {
-      const int k0 = op[1], k1 = op[2], c0 = op[3], c1 = op[4];
-      const cplx<T>* iv = inv + (long)(k0 / P) * 2 * P * P + (ty == OP_TRSM_U ? P * P : 0);
-      cplx<T>* X = M + (long)k0 * ldm + c0;
-      wg_gemm<T, LW>(iv, P, X, ldm, X, ldm, k1 - k0, c1 - c0, k1 - k0, true, L);
-    } else if (ty == OP_GEMM) {
-      const int r0 = op[1], r1 = op[2], c0 = op[3], c1 = op[4], k0 = op[5], k1 = op[6];
-      wg_gemm<T, LW>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm, M + (long)r0 * ldm + c0, ldm,
-                     r1 - r0, c1 - c0, k1 - k0, false, L);
-    } else {  // OP_SWAP_RHS
-      for (int c = N + threadIdx.x; c < W; c += NT)
-        for (int j = 0; j < N; ++j) {
-          int p = piv[j];
-          if (p != j) {
-            cplx<T> u = M[(long)j * ldm + c];
-            M[(long)j * ldm + c] = M[(long)p * ldm + c];
-            M[(long)p * ldm + c] = u;
+  wg_panel<T, RPT, LW>(Mall + (long)mat * mstride, ldm, N, W, c0, c1, pivall + (long)mat * N,
+                       invall + (long)mat * istride, L, myinfo);
+  if (threadIdx.x == 0 && myinfo != 0 && info[mat] == 0) info[mat] = myinfo;
+}
+
+__global__ void k_lu_init(int nmat, int* __restrict__ info) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nmat) info[i] = 0;
+}
+
+// The pivot sequence in `piv` (LAPACK's ipiv, 0-based) is replaced by the row permutation it
+// amounts to: row j of the factors is row piv[j] of the input, which is what the solve gathers with.
+constexpr int PERM_MAX_N = 4096;
+__global__ void __launch_bounds__(NT) k_lu_perm(int N, int* __restrict__ pivall) {
+  __shared__ int pl[PERM_MAX_N], ps[PERM_MAX_N];
+  int* piv = pivall + (long)blockIdx.x * N;
+  for (int j = threadIdx.x; j < N; j += NT) {
+    pl[j] = j;
+    ps[j] = piv[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < N; ++j) {
+      const int p = ps[j];
+      if (p != j) {
+        const int u = pl[j];
+        pl[j] = pl[p];
+        pl[p] = u;
+      }
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < N; j += NT) piv[j] = pl[j];
+}
+
+// ---------------------------------------------------------------- the solve kernel
+// Y (+)= (P^T L U)^-1 R for the ns right-hand-side columns [blockIdx.y*ns, ...) of matrix
+// blockIdx.x, R and Y in FP64 with layout [matrix][row][nrhs].  The columns are gathered through
+// the permutation into LDS (in the factor precision) and stay there; the factors stream from HBM
+// once: block row by block row (P rows), the off-diagonal part as dot products against the
+// resident solution (16 lanes per row, shuffle reduction), the diagonal block through its stored
+// inverse.  Wave w owns rows 8w .. 8w+7 of a block row, lane l rows 8w + l/16 and 8w + 4 + l/16 at
+// columns l%16 + 16j.
+constexpr int SOLVE_LDS_BYTES = 80896;  // two workgroups per CU
+
+template <typename T, int CG>
+__global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ldm, const cplx<T>* __restrict__ Mall,
+                                                const int* __restrict__ permall, const cplx<T>* __restrict__ invall,
+                                                const z64* __restrict__ R, z64* __restrict__ Y, int accumulate) {
+  using C = cplx<T>;
+  __shared__ C x[SOLVE_LDS_BYTES / sizeof(C)];  // x[j * ns + s]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
+  const int m = blockIdx.x, s0 = blockIdx.y * ns, nsg = min(ns, nrhs - s0);
+  const C* M = Mall + (long)m * N * ldm;
+  const int* perm = permall + (long)m * N;
+  const int nb = (N + P - 1) / P;
+  const C* inv = invall + (long)m * nb * 2 * P * P;
+  const z64* Rm = R + (long)m * N * nrhs + s0;
+  z64* Ym = Y + (long)m * N * nrhs + s0;
+  for (int e = t; e < N * nsg; e += NT) {
+    const int j = e / nsg, s = e % nsg;
+    x[j * ns + s] = ccast<T>(Rm[(long)perm[j] * nrhs + s]);
+  }
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {  // 0: forward with unit-lower L, 1: backward with U
+    for (int q = 0; q < nb; ++q) {
+      const int ib = pass == 0 ? q : nb - 1 - q;
+      const int r0 = ib * P, r1 = min(N, r0 + P);
+      const int k0 = pass == 0 ? 0 : r1, k1 = pass == 0 ? r0 : N;
+      if (k1 > k0) {
+        const int ra = r0 + wave * 8 + rs, rb = ra + 4;
+        const C* La = M + (long)min(ra, N - 1) * ldm;
+        const C* Lb = M + (long)min(rb, N - 1) * ldm;
+        C acc0[CG], acc1[CG];
+#pragma unroll
+        for (int s = 0; s < CG; ++s) acc0[s] = acc1[s] = mk<T>(0, 0);
+        // software pipeline: the next group of SU loads per row is in flight while this one is used
+        constexpr int SU = 8;
+        C pa0[SU], pa1[SU], qa0[SU], qa1[SU];
+        auto load = [&](C* d0, C* d1, int kb) {
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            const int kk = kb + 16 * u;
+            const bool v = kk < k1;
+            d0[u] = v ? La[kk] : mk<T>(0, 0);
+            d1[u] = v ? Lb[kk] : mk<T>(0, 0);
+          }
+        };
+        auto use = [&](const C* d0, const C* d1, int kb) {
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            const int kk = kb + 16 * u;
+            if (kk < k1) {
+              const C* xr = x + kk * ns;
+#pragma unroll
+              for (int s = 0; s < CG; ++s)
+                if (s < nsg) {
+                  const C xv = xr[s];
+                  acc0[s] = cfma(d0[u], xv, acc0[s]);
+                  acc1[s] = cfma(d1[u], xv, acc1[s]);
+                }
+            }
+          }
+        };
+        int k = k0 + ks;
+        load(pa0, pa1, k);
+        while (k < k1) {
+          int kn = k + 16 * SU;
+          if (kn < k1) load(qa0, qa1, kn);
+          use(pa0, pa1, k);
+          k = kn;
+          if (k >= k1) break;
+          kn = k + 16 * SU;
+          if (kn < k1) load(pa0, pa1, kn);
+          use(qa0, qa1, k);
+          k = kn;
+        }
+#pragma unroll
+        for (int s = 0; s < CG; ++s) {
+#pragma unroll
+          for (int off = 8; off > 0; off >>= 1) {
+            acc0[s].re += __shfl_xor(acc0[s].re, off);
+            acc0[s].im += __shfl_xor(acc0[s].im, off);
+            acc1[s].re += __shfl_xor(acc1[s].re, off);
+            acc1[s].im += __shfl_xor(acc1[s].im, off);
           }
         }
+        if (ks == 0) {
+#pragma unroll
+          for (int s = 0; s < CG; ++s)
+            if (s < nsg) {
+              if (ra < r1) x[ra * ns + s] = csub(x[ra * ns + s], acc0[s]);
+              if (rb < r1) x[rb * ns + s] = csub(x[rb * ns + s], acc1[s]);
+            }
+        }
+      }
+      __syncthreads();
+      // diagonal block: x_b <- D^-1 x_b with D^-1 the stored inverse (P x P, zero beyond the block)
+      const C* Di = inv + (long)ib * 2 * P * P + (pass == 0 ? 0 : P * P);
+      C o[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = t + h * NT;
+        o[h] = mk<T>(0, 0);
+        if (e < P * nsg) {
+          const int r = e / nsg, s = e % nsg;
+          if (r0 + r < r1) {
+            const C* drow = Di + r * P;
+            const int c0 = pass == 0 ? 0 : r, c1 = pass == 0 ? r + 1 : r1 - r0;
+            C a = mk<T>(0, 0);
+            for (int c = c0; c < c1; ++c) a = cfma(drow[c], x[(r0 + c) * ns + s], a);
+            o[h] = a;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = t + h * NT;
+        if (e < P * nsg) {
+          const int r = e / nsg, s = e % nsg;
+          if (r0 + r < r1) x[(r0 + r) * ns + s] = o[h];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && info) info[blockIdx.x] = myinfo;
+  for (int e = t; e < N * nsg; e += NT) {
+    const int j = e / nsg, s = e % nsg;
+    const C v = x[j * ns + s];
+    z64 y = mk((double)v.re, (double)v.im);
+    if (accumulate) y = cadd(Ym[(long)j * nrhs + s], y);
+    Ym[(long)j * nrhs + s] = y;
+  }
 }
 
 // ---------------------------------------------------------------- host: operation lists
@@ -551,67 +583,146 @@ std::vector<int> lu_ops_factor(int N, int W) {
   return ops.v;
 }
 
-// solve only, with the factors in place and new right-hand sides in columns [N, W)
-std::vector<int> lu_ops_solve(int N, int W) {
-  Ops ops;
-  ops.add(OP_SWAP_RHS, 0, 0, 0, 0);
-  rec_trsm_l(ops, 0, N, N, W);
-  rec_trsm_u(ops, 0, N, N, W);
-  return ops.v;
+namespace {
+
+template <typename T, int TMT, int TNT, int WR, int WC>
+void gemm_launch(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
+                 const cplx<T>* inv, long istride, hipStream_t s) {
+  const int ntr = (m + TMT - 1) / TMT, ntc = (n + TNT - 1) / TNT, ntile = ntr * ntc;
+  const long nwg = (long)ntile * ((nmat + 7) / 8) * 8;
+  hipLaunchKernelGGL((k_gemm<T, TMT, TNT, WR, WC>), dim3((unsigned)nwg), dim3(NT), 0, s, nmat, ntc, ntile, mstride, M,
+                     ldm, r0, m, c0, n, k0, K, mode, inv, istride);
 }
 
+// tile shape by operand shape: the smallest row/column tile (32, 64, 128) covering m / n, so narrow
+// and short updates do not pay for empty MFMA blocks; 128x128 tiles (64x64 per wave) for the large
+// Schur updates halve their re-reads of A and B.  complex128 stops at 128x64 (registers).
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, int nops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s) {
-  int rpt = round_rpt((N + NT - 1) / NT);
-  dim3 grid(nmat), blk(NT);
-#define ADGA_LU_CASE(R)                                                                                    \
-  case R:                                                                                                  \
-    hipLaunchKernelGGL((k_lu<T, R, LeafW<T, R>::value>), grid, blk, 0, s, N, W, ldm, nops, ops, M, piv, inv, \
-                       info);                                                                              \
-    return true;
-  switch (rpt) {
-    ADGA_LU_CASE(1)
-    ADGA_LU_CASE(2)
-    ADGA_LU_CASE(3)
-    ADGA_LU_CASE(4)
-    ADGA_LU_CASE(5)
-    ADGA_LU_CASE(6)
-    ADGA_LU_CASE(8)
-    ADGA_LU_CASE(12)
-    ADGA_LU_CASE(16)
-    default: return false;
+void gemm_op(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
+             const cplx<T>* inv, long istride, hipStream_t s, LuTimer* timer) {
+  const int tm = m > 64 ? 128 : m > 32 ? 64 : 32;
+  const int tn = n > 64 ? 128 : n > 32 ? 64 : 32;
+  // the largest tile shape of this precision is the one the timer brackets with events
+  const bool timed = timer && tm == 128 && tn == 128;
+  if (timed) {
+    timer->mark(s);
+    timer->flops += 8.0 * m * (double)n * K * nmat;
   }
-#undef ADGA_LU_CASE
+#define ADGA_G(TMT, TNT, WR, WC) \
+  gemm_launch<T, TMT, TNT, WR, WC>(nmat, mstride, M, ldm, r0, m, c0, n, k0, K, mode, inv, istride, s)
+  if (tm == 128 && tn == 128) {
+    if constexpr (sizeof(T) == 4) ADGA_G(128, 128, 2, 2);
+    else ADGA_G(128, 64, 2, 2);
+  } else if (tm == 128 && tn == 64) {
+    ADGA_G(128, 64, 2, 2);
+  } else if (tm == 128) {
+    ADGA_G(128, 32, 4, 1);
+  } else if (tm == 64 && tn == 128) {
+    if constexpr (sizeof(T) == 4) ADGA_G(64, 128, 2, 2);
+    else ADGA_G(64, 64, 2, 2);
+  } else if (tm == 32 && tn == 128) {
+    ADGA_G(32, 128, 1, 4);
+  } else if (tm == 32 && tn == 32) {
+    ADGA_G(32, 32, 2, 2);
+  } else {
+    ADGA_G(64, 64, 2, 2);
+  }
+#undef ADGA_G
+  if (timed) timer->mark(s);
 }
-template bool launch_lu<float>(int, int, int, int, const int*, int, c32*, int*, c32*, int*, hipStream_t);
-template bool launch_lu<double>(int, int, int, int, const int*, int, z64*, int*, z64*, int*, hipStream_t);
 
-// Y[m][r][s] (FP64) = M_m[r][N + s]  (stage 0),  Y += M_m[r][N + s]  (stage 1),
-// M_m[r][N + s] = R[m][r][s] rounded to T  (stage 2)
+}  // namespace
+
+// The factorisation: the host walks the operation list and launches one batched kernel per
+// operation (panels: a workgroup per matrix; GEMM/TRSM: a workgroup per output tile per matrix).
 template <typename T>
-__global__ void k_rhs_io(int N, int nrhs, int ldm, int nmat, int stage, cplx<T>* __restrict__ M, z64* __restrict__ Y,
-                         const z64* __restrict__ R) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long tot = (long)nmat * N * nrhs;
-  if (idx >= tot) return;
-  int s = (int)(idx % nrhs);
-  long mr = idx / nrhs;  // m*N + r
-  cplx<T>* e = M + mr * ldm + N + s;
-  if (stage == 0)
-    Y[idx] = ccast<double>(*e);
-  else if (stage == 1)
-    Y[idx] = cadd(Y[idx], ccast<double>(*e));
-  else
-    *e = ccast<T>(R[idx]);
+bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
+               int* info, hipStream_t s, LuTimer* timer) {
+  const int rpt = round_rpt((N + NT - 1) / NT);
+  if (rpt < 0 || N > PERM_MAX_N || nmat < 1) return false;
+  const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
+  hipLaunchKernelGGL(k_lu_init, dim3((nmat + 255) / 256), dim3(256), 0, s, nmat, info);
+  for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
+    const int* op = &ops[o];
+    if (op[0] == OP_PANEL) {
+      switch (rpt) {
+#define ADGA_P(R)                                                                                             \
+  case R:                                                                                                     \
+    hipLaunchKernelGGL((k_panel<T, R, LeafW<T, R>::value>), dim3(nmat), dim3(NT), 0, s, N, W, ldm, mstride, op[1], \
+                       op[2], M, piv, inv, istride, info);                                                    \
+    break;
+        ADGA_P(1) ADGA_P(2) ADGA_P(3) ADGA_P(4) ADGA_P(5) ADGA_P(6) ADGA_P(8) ADGA_P(12) ADGA_P(16)
+#undef ADGA_P
+        default: return false;
+      }
+    } else if (op[0] == OP_TRSM_L || op[0] == OP_TRSM_U) {
+      const int K = op[2] - op[1];
+      gemm_op<T>(nmat, mstride, M, ldm, 0, K, op[3], op[4] - op[3], op[1], K, op[0] == OP_TRSM_L ? 1 : 2, inv,
+                 istride, s, nullptr);
+    } else {
+      gemm_op<T>(nmat, mstride, M, ldm, op[1], op[2] - op[1], op[3], op[4] - op[3], op[5], op[6] - op[5], 0, inv,
+                 istride, s, timer);
+    }
+  }
+  hipLaunchKernelGGL(k_lu_perm, dim3(nmat), dim3(NT), 0, s, N, piv);
+  return true;
 }
+template bool launch_lu<float>(int, int, int, int, const std::vector<int>&, c32*, int*, c32*, int*, hipStream_t,
+                               LuTimer*);
+template bool launch_lu<double>(int, int, int, int, const std::vector<int>&, z64*, int*, z64*, int*, hipStream_t,
+                                LuTimer*);
+
+void LuTimer::mark(hipStream_t s) {
+  if (used == ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ev.push_back(e);
+  }
+  (void)hipEventRecord(ev[used++], s);
+}
+
+double LuTimer::collect() {
+  double ms = 0.0;
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) ms += t;
+  }
+  launches = (int)(used / 2);
+  used = 0;
+  return ms;
+}
+
+LuTimer::~LuTimer() {
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+
+// columns per group: as many as the LDS holds (at most 16); false when not even one column fits
 template <typename T>
-void launch_rhs_io(int nmat, int N, int nrhs, int ldm, int stage, cplx<T>* M, z64* Y, const z64* R, hipStream_t s) {
-  long tot = (long)nmat * N * nrhs;
-  hipLaunchKernelGGL(k_rhs_io<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nrhs, ldm, nmat, stage, M,
-                     Y, R);
+bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s) {
+  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / N;
+  int ns = std::min(std::min(nrhs, cap), 16);
+  if (ns < 1) return false;
+  const int ng = (nrhs + ns - 1) / ns;
+  ns = (nrhs + ng - 1) / ng;  // balance the groups
+  dim3 grid(nmat, ng);
+#define ADGA_SOLVE_CASE(CGV)                                                                             \
+  if (ns <= CGV) {                                                                                       \
+    hipLaunchKernelGGL((k_lu_solve<T, CGV>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
+                       accumulate ? 1 : 0);                                                              \
+    return true;                                                                                         \
+  }
+  ADGA_SOLVE_CASE(1)
+  ADGA_SOLVE_CASE(2)
+  ADGA_SOLVE_CASE(4)
+  ADGA_SOLVE_CASE(9)
+  ADGA_SOLVE_CASE(16)
+#undef ADGA_SOLVE_CASE
+  return false;
 }
-template void launch_rhs_io<float>(int, int, int, int, int, c32*, z64*, const z64*, hipStream_t);
-template void launch_rhs_io<double>(int, int, int, int, int, z64*, z64*, const z64*, hipStream_t);
+template bool launch_lu_solve<float>(int, int, int, int, const c32*, const int*, const c32*, const z64*, z64*, bool,
+                                     hipStream_t);
+template bool launch_lu_solve<double>(int, int, int, int, const z64*, const int*, const z64*, const z64*, z64*, bool,
+                                      hipStream_t);
 
 }  // namespace adga

@@ -113,9 +113,12 @@ int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *
 int adga_reset_sigma(adga_ctx *ctx);
 
 /*
- * Statistics of the last adga_run_points call: out[0] = max refinement iterations,
- * out[1] = max final relative residual, out[2..8] = milliseconds spent in the reference's stage
- * buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM, Chi (src/main.F90:484-783).
+ * Statistics of the last adga_run_points call (nout <= 14 values): out[0] = max refinement
+ * iterations, out[1] = max final relative residual, out[2..8] = milliseconds spent in the
+ * reference's stage buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM, Chi
+ * (src/main.F90:484-783), out[9] = milliseconds of the batched LU factorisations, out[10] = their
+ * number, out[11..13] = milliseconds, algorithmic flops and launches of the factorisation's
+ * dominant GEMM kernel (the largest tile shape).
  */
 int adga_get_stats(adga_ctx *ctx, double *out, int nout);
 
