@@ -97,42 +97,55 @@ void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64
 
 // ---------------------------------------------------------------- A6: BSE build
 // M_{b,ch}(r, d*n2 + s) = delta - R_d(s, r),  R_d(s, r) = sum_i' S_b[d][s][i'] C_ch(d*n2 + i', r)
-// (main.F90:693-700, transposed: M = A^T).  A workgroup owns DB frequency blocks d of one
-// (point, channel) for all rows: S_d sits in LDS, and each thread turns the n2 contiguous values
-// C(d*n2 + ., r) (a row slice of C^T) into n2 outputs.  Padding columns [N, ldm) are not written
-// (never read).
+// (main.F90:693-700, transposed: M = A^T).  A workgroup owns DB frequency blocks d of one channel
+// for PB points and all rows: their S_d sit in LDS, and each thread reads the n2 contiguous values
+// C(d*n2 + ., r) (a row slice of C^T) once and turns them into n2 outputs per point.  Padding
+// columns [N, ldm) are not written (never read).
+template <int NN>
+struct BuildShape {
+  static constexpr int N2 = NN * NN, N4 = N2 * N2;
+  static constexpr int DB = NN <= 3 ? 8 : 2;  // frequency blocks per workgroup
+  static constexpr int PB0 = 40960 / (DB * N4 * 16);
+  static constexpr int PB = PB0 < 1 ? 1 : PB0 > 4 ? 4 : PB0;  // points per workgroup (share each C row read)
+};
+
 template <typename T, int NN>
-__global__ void __launch_bounds__(256) k_build(Geom g, int ldm, const z64* __restrict__ S, const z64* __restrict__ c_d,
-                                               const z64* __restrict__ c_m, cplx<T>* __restrict__ M) {
-  constexpr int N2 = NN * NN, N4 = N2 * N2;
-  constexpr int DB = NN <= 3 ? 8 : 2;
-  __shared__ z64 sS[DB * N4];
-  const int ch = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
+__global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const z64* __restrict__ S,
+                                               const z64* __restrict__ c_d, const z64* __restrict__ c_m,
+                                               cplx<T>* __restrict__ M) {
+  using Sh = BuildShape<NN>;
+  constexpr int N2 = Sh::N2, N4 = Sh::N4, DB = Sh::DB, PB = Sh::PB;
+  __shared__ z64 sS[PB][DB * N4];
+  const int ch = blockIdx.y, b0 = blockIdx.z * PB, t = threadIdx.x;
   const int N = g.N;
-  cplx<T>* Mb = M + (long)(2 * b + ch) * N * ldm;
   const int d0 = blockIdx.x * DB;
   const int nd = min(DB, g.nf - d0);
-  const z64* Sb = S + ((long)b * g.nf + d0) * N4;
-  for (int e = t; e < nd * N4; e += 256) sS[e] = Sb[e];
+  const int np = min(PB, npts - b0);
+  for (int pb = 0; pb < np; ++pb) {
+    const z64* Sb = S + ((long)(b0 + pb) * g.nf + d0) * N4;
+    for (int e = t; e < nd * N4; e += 256) sS[pb][e] = Sb[e];
+  }
   __syncthreads();
   const z64* C = ch == 0 ? c_d : c_m;
   const int dl = t % DB, rl = t / DB;
   if (dl >= nd) return;
   const int d = d0 + dl;
-  const z64* sd = sS + dl * N4;
   for (int r = rl; r < N; r += 256 / DB) {
     z64 cc[N2];
     const z64* crow = C + (long)r * N + d * N2;
 #pragma unroll
     for (int i = 0; i < N2; ++i) cc[i] = crow[i];
-    cplx<T>* mrow = Mb + (long)r * ldm + d * N2;
-    // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
+    for (int pb = 0; pb < np; ++pb) {
+      const z64* sd = sS[pb] + dl * N4;
+      cplx<T>* mrow = M + ((long)(2 * (b0 + pb) + ch) * N + r) * ldm + d * N2;
+      // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
 #pragma unroll 1
-    for (int sq = 0; sq < N2; ++sq) {
-      z64 acc = mk(0.0, 0.0);
+      for (int sq = 0; sq < N2; ++sq) {
+        z64 acc = mk(0.0, 0.0);
 #pragma unroll
-      for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
-      mrow[sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
+        for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
+        mrow[sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
+      }
     }
   }
 }
@@ -142,9 +155,9 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
                   hipStream_t s) {
 #define ADGA_BUILD_CASE(NN)                                                                            \
   case NN: {                                                                                           \
-    constexpr int DB = NN <= 3 ? 8 : 2;                                                                \
-    dim3 grid((g.nf + DB - 1) / DB, 2, npts);                                                         \
-    hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, ldm, S, cd, cm, M);                \
+    constexpr int DB = BuildShape<NN>::DB, PB = BuildShape<NN>::PB;                                   \
+    dim3 grid((g.nf + DB - 1) / DB, 2, (npts + PB - 1) / PB);                                         \
+    hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, npts, ldm, S, cd, cm, M);          \
     break;                                                                                             \
   }
   switch (g.n) {

@@ -69,25 +69,32 @@ struct PanelLds {
       cplx<T> Xb[2][P][P + 1];  // its inverses being formed (L, U)
     } v;
   } u;
-  cplx<T> sP[LW], sQ[LW];          // pivot row exchange
-  cplx<T> Xs[2][LW][LW + 1];       // left-looking: a previous leaf's rows of the current leaf
+  // pivot search, double-buffered by column parity: each wave's best candidate (|re|+|im|, row) with
+  // the row's leaf values, and the values of the row the pivot replaces
+  cplx<T> cv[2][NT / 64][LW];
+  T cval[2][NT / 64];
+  int crow[2][NT / 64];
+  cplx<T> kv[2][LW];
+  cplx<T> Xs[2][LW][LW + 1];       // left-looking: [0] raw rows of the current leaf, [1] their U block
   cplx<T> Li[P / LW][LW][LW + 1];  // unit-lower inverses of the panel's leaves
-  T redv[NT / 64];
-  int redi[NT / 64];
   int spiv[LW];
 };
 
 // ---------------------------------------------------------------- panel factorisation
-// Columns [c0, c1) of rows [c0, N); thread t holds rows c0 + t + i*NT (i < RPT) of the current leaf.
+// Columns [c0, c1) of rows [c0, N); thread t holds rows c0 + t + i*NT (i < RPT) of the current leaf
+// of LW columns.  Barriers: two per (leaf, earlier leaf) pair of the left-looking updates, one per
+// column for the pivot (each wave publishes its best candidate together with that row's values, so
+// the winner's row needs no second round), two per leaf.
 template <typename T, int RPT, int LW>
 __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N, int W, int c0, int c1,
                                          int* __restrict__ piv, cplx<T>* __restrict__ inv, PanelLds<T, LW>& L,
                                          int& myinfo) {
   using C = cplx<T>;
   constexpr int NWAVE = NT / 64;
+  constexpr int NOROW = 0x7fffffff;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nl = (c1 - c0 + LW - 1) / LW;
-  int xb = 0;  // double-buffer index of Xs
+  int q = 0;  // parity of the pivot-search buffers
   for (int jl = 0; jl < nl; ++jl) {
     const int j0 = c0 + jl * LW;
     const int w = min(LW, c1 - j0);
@@ -103,107 +110,118 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
       const int p0 = c0 + il * LW;
       const int a = t - (p0 - c0);  // this thread owns row p0 + a (slot 0) when 0 <= a < LW
       const bool owner = a >= 0 && a < LW;
+      C lrow[RPT][LW];  // L values of the rows below the earlier leaf, loaded ahead of the barriers
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int r = c0 + t + i * NT;
+        const bool act = r >= p0 + LW && r < N;
+#pragma unroll
+        for (int b = 0; b < LW; ++b) lrow[i][b] = act ? M[(long)r * ldm + p0 + b] : mk<T>(0, 0);
+      }
       if (owner) {
 #pragma unroll
-        for (int j = 0; j < LW; ++j) L.Xs[xb][a][j] = reg[0][j];
+        for (int j = 0; j < LW; ++j) L.Xs[0][a][j] = reg[0][j];
       }
       __syncthreads();
-      C xn[LW];
-      if (owner) {  // rows of the U block: L_ii^-1 applied
+      if (owner) {  // row a of the U block: L_ii^-1 applied to the raw rows
+        C xn[LW];
 #pragma unroll
         for (int j = 0; j < LW; ++j) {
           C acc = mk<T>(0, 0);
 #pragma unroll
           for (int b = 0; b < LW; ++b)
-            if (b <= a) acc = cfma(L.Li[il][a][b], L.Xs[xb][b][j], acc);
+            if (b <= a) acc = cfma(L.Li[il][a][b], L.Xs[0][b][j], acc);
           xn[j] = acc;
         }
-      }
-      __syncthreads();
-      if (owner) {
 #pragma unroll
         for (int j = 0; j < LW; ++j) {
           reg[0][j] = xn[j];
-          L.Xs[xb][a][j] = xn[j];
+          L.Xs[1][a][j] = xn[j];
         }
       }
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
-        int r = c0 + t + i * NT;
+        const int r = c0 + t + i * NT;
         if (r >= p0 + LW && r < N) {
-          const C* lrow = M + (long)r * ldm + p0;
 #pragma unroll
-          for (int b = 0; b < LW; ++b) {
-            C l = lrow[b];
+          for (int b = 0; b < LW; ++b)
 #pragma unroll
-            for (int j = 0; j < LW; ++j) reg[i][j] = cfms(l, L.Xs[xb][b][j], reg[i][j]);
-          }
+            for (int j = 0; j < LW; ++j) reg[i][j] = cfms(lrow[i][b], L.Xs[1][b][j], reg[i][j]);
         }
       }
-      xb ^= 1;
     }
-    // ---- factor the leaf (rows >= j0 are pivot candidates)
+    // ---- factor the leaf (rows >= kc are pivot candidates)
 #pragma unroll
     for (int j = 0; j < LW; ++j) {
       if (j < w) {
         const int kc = j0 + j;
         T bv = (T)-1;
-        int br = 0x7fffffff;
+        int br = NOROW, bs = 0;
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
-          int r = c0 + t + i * NT;
+          const int r = c0 + t + i * NT;
           if (r < N && r >= kc) {
-            T v = cabs1(reg[i][j]);
-            if (v > bv) { bv = v; br = r; }
+            const T v = cabs1(reg[i][j]);
+            if (v > bv) { bv = v; br = r; bs = i; }
           }
         }
+        T wv = bv;
+        int wr = br;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-          T ov = __shfl_xor(bv, off);
-          int orow = __shfl_xor(br, off);
-          if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
+          const T ov = __shfl_xor(wv, off);
+          const int orow = __shfl_xor(wr, off);
+          if (ov > wv || (ov == wv && orow < wr)) { wv = ov; wr = orow; }
         }
-        if (lane == 0) { L.redv[wave] = bv; L.redi[wave] = br; }
-        __syncthreads();
-        bv = L.redv[0];
-        br = L.redi[0];
-#pragma unroll
-        for (int q = 1; q < NWAVE; ++q) {
-          T ov = L.redv[q];
-          int orow = L.redi[q];
-          if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
-        }
-        // no finite candidate (NaN column): keep the diagonal row and flag the matrix
-        const int p = (br < N) ? br : kc;
-        if (br >= N && myinfo == 0) myinfo = kc + 1;
-        const int tp = (p - c0) % NT, sp = (p - c0) / NT;
-        const int tk = (kc - c0) % NT, sk = (kc - c0) / NT;
-        if (t == tp) {
+        if (wr != NOROW && br == wr) {  // the lane holding the wave's best row publishes it
+          L.cval[q][wave] = wv;
+          L.crow[q][wave] = wr;
 #pragma unroll
           for (int i = 0; i < RPT; ++i)
-            if (i == sp) {
+            if (i == bs) {
 #pragma unroll
-              for (int jj = 0; jj < LW; ++jj) L.sP[jj] = reg[i][jj];
+              for (int jj = 0; jj < LW; ++jj) L.cv[q][wave][jj] = reg[i][jj];
             }
+        } else if (wr == NOROW && lane == 0) {
+          L.cval[q][wave] = (T)-1;
+          L.crow[q][wave] = NOROW;
         }
-        if (p != kc && t == tk) {
+        const int tk = (kc - c0) % NT, sk = (kc - c0) / NT;
+        if (t == tk) {
 #pragma unroll
           for (int i = 0; i < RPT; ++i)
             if (i == sk) {
 #pragma unroll
-              for (int jj = 0; jj < LW; ++jj) L.sQ[jj] = reg[i][jj];
+              for (int jj = 0; jj < LW; ++jj) L.kv[q][jj] = reg[i][jj];
             }
         }
-        if (t == 0) { L.spiv[j] = p; piv[kc] = p; }
         __syncthreads();
+        T pbv = L.cval[q][0];
+        int p = L.crow[q][0], pw = 0;
+#pragma unroll
+        for (int w2 = 1; w2 < NWAVE; ++w2) {
+          const T ov = L.cval[q][w2];
+          const int orow = L.crow[q][w2];
+          if (ov > pbv || (ov == pbv && orow < p)) { pbv = ov; p = orow; pw = w2; }
+        }
+        // no finite candidate (NaN column): keep the diagonal row and flag the matrix
+        if (p >= N) {
+          p = kc;
+          if (myinfo == 0) myinfo = kc + 1;
+        }
+        C pr[LW];
+#pragma unroll
+        for (int jj = 0; jj < LW; ++jj) pr[jj] = (p == kc) ? L.kv[q][jj] : L.cv[q][pw][jj];
+        if (t == 0) { L.spiv[j] = p; piv[kc] = p; }
         if (p != kc) {
+          const int tp = (p - c0) % NT, sp = (p - c0) / NT;
           if (t == tp) {
 #pragma unroll
             for (int i = 0; i < RPT; ++i)
               if (i == sp) {
 #pragma unroll
-                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = L.sQ[jj];
+                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = L.kv[q][jj];
               }
           }
           if (t == tk) {
@@ -211,31 +229,32 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
             for (int i = 0; i < RPT; ++i)
               if (i == sk) {
 #pragma unroll
-                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = L.sP[jj];
+                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = pr[jj];
               }
           }
         }
-        const C pv = L.sP[j];
+        const C pv = pr[j];
         if (pv.re == (T)0 && pv.im == (T)0) {
           if (myinfo == 0) myinfo = kc + 1;
         } else {
           const C rp = crecip(pv);
 #pragma unroll
           for (int i = 0; i < RPT; ++i) {
-            int r = c0 + t + i * NT;
+            const int r = c0 + t + i * NT;
             if (r < N && r > kc) {
-              C l = cmul(reg[i][j], rp);
+              const C l = cmul(reg[i][j], rp);
               reg[i][j] = l;
 #pragma unroll
-              for (int jj = j + 1; jj < LW; ++jj) reg[i][jj] = cfms(l, L.sP[jj], reg[i][jj]);
+              for (int jj = j + 1; jj < LW; ++jj) reg[i][jj] = cfms(l, pr[jj], reg[i][jj]);
             }
           }
         }
+        q ^= 1;
       }
     }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
-      int r = c0 + t + i * NT;
+      const int r = c0 + t + i * NT;
       if (r < N) {
 #pragma unroll
         for (int j = 0; j < LW; ++j)
@@ -247,7 +266,7 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
       const int a = t - (j0 - c0);
       if (a >= 0 && a < LW) {
 #pragma unroll
-        for (int j = 0; j < LW; ++j) L.Xs[xb][a][j] = reg[0][j];
+        for (int j = 0; j < LW; ++j) L.Xs[0][a][j] = reg[0][j];
       }
     }
     __syncthreads();
@@ -255,7 +274,7 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
       for (int a = 0; a < LW; ++a) {
         C y = mk<T>(a == t ? 1 : 0, 0);
         if (a > t && a < w)
-          for (int b = t; b < a; ++b) y = cfms(L.Xs[xb][a][b], L.Li[jl][b][t], y);
+          for (int b = t; b < a; ++b) y = cfms(L.Xs[0][a][b], L.Li[jl][b][t], y);
         L.Li[jl][a][t] = (a < w && t < w) ? y : mk<T>(0, 0);
       }
     }
@@ -276,7 +295,6 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
         }
       }
     }
-    xb ^= 1;
     __syncthreads();
   }
   // ---- inverses of the panel's L11 (unit lower) and U11 (upper) into the side buffer
@@ -560,9 +578,10 @@ void rec_lu(Ops& ops, int N, int W, int c0, int c1, bool rhs) {
 
 }  // namespace
 
+// leaf width: 8 columns in registers for complex64 up to 6 rows per thread, else 4
 template <typename T, int RPT>
 struct LeafW {
-  static constexpr int value = 4;
+  static constexpr int value = (sizeof(T) == 4 && RPT <= 6) ? 8 : 4;
 };
 
 static int round_rpt(int rpt) {
