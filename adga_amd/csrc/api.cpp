@@ -23,7 +23,7 @@ struct adga_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int refine_maxit = 12;
-  double refine_tol = 1e-13;
+  double refine_tol = 1e-12;
   int batch = 256;
   int ldm = 0;
   // one-particle data (device)

