@@ -66,13 +66,7 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
     for s in range(warmup):
         step(s)
     ctx.reset_sigma()
-    cur[0] = None
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    cur[0] = None  # run_points returns with its stream drained, so the device is idle here
     lu_ms, lu_n, iters, resid = 0.0, 0, 0.0, 0.0
     g_ms, g_flops, g_n = 0.0, 0.0, 0
     outs = []
@@ -154,17 +148,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first copy 1 GiB device-to-device (a kernel of known HBM traffic, so that hardware "
-                         "byte counters collected over this run can be scaled)")
+                    help="only copy 1 GiB device-to-device and exit: a kernel of known HBM traffic whose "
+                         "hardware byte counters scale those of the hot-path kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path is built and measured on a single MI355X; "
               "sharding across GPUs is a design note only (--gpus 1).", file=sys.stderr)
         sys.exit(2)
 
-    from adga_amd import api, synth
     if args.copy_calibration:
         copy_calibration()
+        return
+    from adga_amd import api, synth
     wl = synth.workload(args.workload)
     inp = synth.build_inputs(wl)
     omegas = [0, 1, -1, 2, -2, 3, -3]
