@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
                                                                                   long mstride, cplx<T>* __restrict__ Mall,
                                              int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
                                              const cplx<T>* __restrict__ invall, long istride) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[gemm_lds_bytes<T>()];
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RmCfg<T, TMT, TNT>::LDS];
   const int L = blockIdx.x, x = L & 7, sl = L >> 3;
   const int mat = (sl / ntile) * 8 + x, tile = sl % ntile;
   if (mat >= nmat) return;

@@ -152,16 +152,142 @@ __device__ __forceinline__ void gemm_tile_acc(AL aload, BL bload, CL cload, CS c
 }
 
 
-// row-major operands
+// ---------------------------------------------------------------- row-major tile (the LU's GEMMs)
+// Same product as gemm_tile_acc for row-major A, B, C, with the staging tuned for that case:
+// 16-byte global loads (two complex64 or one complex128 per load) without bounds checks on
+// interior tiles of aligned operands, int offsets, and LDS layouts read without bank conflicts by
+// the MFMA fragments: A as As[row][k] (row stride KC complex + 16 bytes), B as Bs[k][col] (row
+// stride TNT complex + 128 bytes, so the lanes of the two K slots fall in the two bank halves).
+template <typename T, int TMT, int TNT>
+struct RmCfg {
+  static constexpr int CS = (int)sizeof(cplx<T>);
+  static constexpr int VEC = 16 / CS;  // complex per 16-byte load
+  static constexpr int budget = 36864;
+  static constexpr int per_k = TMT * CS + (TNT * CS + 128);  // A column + B row bytes per k
+  static constexpr int KC0 = (budget - TMT * 16) / per_k;
+  static constexpr int KC = KC0 >= 32 ? 32 : KC0 >= 16 ? 16 : 8;  // keeps the per-thread load counts whole
+  static constexpr int LDA = KC + 16 / CS;    // As row stride (complex)
+  static constexpr int LDB = TNT + 128 / CS;  // Bs row stride (complex)
+  static constexpr int LDS = (TMT * LDA + KC * LDB) * CS;
+};
+
 template <typename T, int TMT, int TNT, int WR, int WC>
 __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda, const cplx<T>* B, int ldb,
                                           cplx<T>* Cm, int ldc, int m, int n, int K, bool assign, int rt, int ct,
                                           unsigned char* lds) {
-  gemm_tile_acc<T, TMT, TNT, WR, WC>([=](int r, int k) { return A[(long)r * lda + k]; },
-                                     [=](int k, int c) { return B[(long)k * ldb + c]; },
-                                     [=](int r, int c) { return Cm[(long)r * ldc + c]; },
-                                     [=](int r, int c, cplx<T> v) { Cm[(long)r * ldc + c] = v; }, m, n, K, assign, rt,
-                                     ct, lds);
+  using V = typename Mfma<T>::V;
+  using C = cplx<T>;
+  using Cfg = RmCfg<T, TMT, TNT>;
+  static_assert(WR * WC == GEMM_NT / 64, "four waves");
+  constexpr int WTM = TMT / WR, WTN = TNT / WC, RB = WTM / 16, CB = WTN / 16;
+  static_assert(RB >= 1 && CB >= 1 && WTM % 16 == 0 && WTN % 16 == 0, "wave tile of 16x16 blocks");
+  constexpr int KC = Cfg::KC, VEC = Cfg::VEC, LDA = Cfg::LDA, LDB = Cfg::LDB;
+  constexpr int AV = KC / VEC, BV = TNT / VEC;  // 16-byte vectors per A row / B row of a chunk
+  constexpr int PA = TMT * AV / GEMM_NT, PB = KC * BV / GEMM_NT;
+  static_assert(KC >= 8 && KC % VEC == 0 && (TMT * AV) % GEMM_NT == 0 && (KC * BV) % GEMM_NT == 0, "K chunk");
+  struct alignas(16) CV { C v[VEC]; };
+  C* As = reinterpret_cast<C*>(lds);
+  C* Bs = As + TMT * LDA;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wr = (wave / WC) * WTM, wc = (wave % WC) * WTN;
+  const int li = lane & 15, lg = lane >> 4, part = lg & 1, kcl = lg >> 1;
+  const T sgn = assign ? (T)1 : (T)-1;
+  // unchecked 16-byte loads need every row start 16-byte aligned and the whole tile inside
+  const bool aligned = ((reinterpret_cast<size_t>(A) | reinterpret_cast<size_t>(B)) % 16 == 0) &&
+                       (lda % VEC == 0) && (ldb % VEC == 0);
+  const bool inside = aligned && rt + TMT <= m && ct + TNT <= n;
+  V accr[RB][CB], acci[RB][CB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
+        C c = (!assign && row < m && col < n) ? Cm[row * ldc + col] : mk<T>(0, 0);
+        accr[rb][cb][v] = c.re;
+        acci[rb][cb][v] = c.im;
+      }
+  CV pa[PA], pb[PB];
+  auto fetch = [&](int kb) {
+    if (inside && kb + KC <= K) {
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        const int e = t + i * GEMM_NT, row = e / AV, k = (e % AV) * VEC;
+        pa[i] = *reinterpret_cast<const CV*>(A + (rt + row) * lda + kb + k);
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int e = t + i * GEMM_NT, k = e / BV, col = (e % BV) * VEC;
+        pb[i] = *reinterpret_cast<const CV*>(B + (kb + k) * ldb + ct + col);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        const int e = t + i * GEMM_NT, row = rt + e / AV, k = kb + (e % AV) * VEC;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) pa[i].v[j] = (row < m && k + j < K) ? A[row * lda + k + j] : mk<T>(0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int e = t + i * GEMM_NT, k = kb + e / BV, col = ct + (e % BV) * VEC;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) pb[i].v[j] = (k < K && col + j < n) ? B[k * ldb + col + j] : mk<T>(0, 0);
+      }
+    }
+  };
+  fetch(0);
+  for (int kb = 0; kb < K; kb += KC) {
+    if (kb) __syncthreads();  // the previous chunk's MFMAs are done with LDS
+#pragma unroll
+    for (int i = 0; i < PA; ++i) {
+      const int e = t + i * GEMM_NT, row = e / AV, k = (e % AV) * VEC;
+      *reinterpret_cast<CV*>(As + row * LDA + k) = pa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      const int e = t + i * GEMM_NT, k = e / BV, col = (e % BV) * VEC;
+      *reinterpret_cast<CV*>(Bs + k * LDB + col) = pb[i];
+    }
+    __syncthreads();
+    const int kn = kb + KC;
+    if (kn < K) fetch(kn);  // in flight during this chunk's MFMAs
+    const int kend = min(KC, K - kb);
+    const C* Ap = As + (wr + li) * LDA + kcl;
+    const C* Bp = Bs + kcl * LDB + wc + li;
+#pragma unroll 2
+    for (int kk = 0; kk < kend; kk += 2) {
+      T ar[RB], ai[RB], bb[CB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const C a = Ap[rb * 16 * LDA + kk];
+        ar[rb] = sgn * (part ? -a.im : a.re);
+        ai[rb] = sgn * (part ? a.re : a.im);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const C b = Bp[kk * LDB + cb * 16];
+        bb[cb] = part ? b.im : b.re;
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          accr[rb][cb] = Mfma<T>::op(ar[rb], bb[cb], accr[rb][cb]);
+          acci[rb][cb] = Mfma<T>::op(ai[rb], bb[cb], acci[rb][cb]);
+        }
+    }
+  }
+  __syncthreads();  // every wave has consumed its operands before the output (may alias B) lands
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
+        if (row < m && col < n) Cm[row * ldc + col] = mk<T>(accr[rb][cb][v], acci[rb][cb][v]);
+      }
 }
 
 }  // namespace adga
