@@ -80,6 +80,32 @@ struct PanelLds {
   int spiv[LW];
 };
 
+// LW consecutive complex of one row with 16-byte accesses (the row slice is 16-byte aligned: the
+// matrix rows and the leaf columns start on multiples of 16 bytes)
+template <typename T, int LW>
+struct RowSlice {
+  static constexpr int PER = 16 / (int)sizeof(cplx<T>);
+  static_assert(LW % PER == 0, "leaf width in 16-byte units");
+  struct alignas(16) V { cplx<T> v[PER]; };
+  static __device__ __forceinline__ void load(const cplx<T>* p, cplx<T> (&d)[LW]) {
+#pragma unroll
+    for (int u = 0; u < LW / PER; ++u) {
+      const V x = reinterpret_cast<const V*>(p)[u];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) d[u * PER + q] = x.v[q];
+    }
+  }
+  static __device__ __forceinline__ void store(cplx<T>* p, const cplx<T> (&d)[LW]) {
+#pragma unroll
+    for (int u = 0; u < LW / PER; ++u) {
+      V x;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) x.v[q] = d[u * PER + q];
+      reinterpret_cast<V*>(p)[u] = x;
+    }
+  }
+};
+
 // ---------------------------------------------------------------- panel factorisation
 // Columns [c0, c1) of rows [c0, N); thread t holds rows c0 + t + i*NT (i < RPT) of the current leaf
 // of LW columns.  Barriers: two per (leaf, earlier leaf) pair of the left-looking updates, one per
@@ -102,8 +128,12 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       int r = c0 + t + i * NT;
+      if (r < N && w == LW) {
+        RowSlice<T, LW>::load(M + (long)r * ldm + j0, reg[i]);
+      } else {
 #pragma unroll
-      for (int j = 0; j < LW; ++j) reg[i][j] = (r < N && j < w) ? M[(long)r * ldm + j0 + j] : mk<T>(0, 0);
+        for (int j = 0; j < LW; ++j) reg[i][j] = (r < N && j < w) ? M[(long)r * ldm + j0 + j] : mk<T>(0, 0);
+      }
     }
     // ---- left-looking updates from the panel's previous leaves
     for (int il = 0; il < jl; ++il) {
@@ -114,9 +144,12 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
         const int r = c0 + t + i * NT;
-        const bool act = r >= p0 + LW && r < N;
+        if (r >= p0 + LW && r < N) {
+          RowSlice<T, LW>::load(M + (long)r * ldm + p0, lrow[i]);
+        } else {
 #pragma unroll
-        for (int b = 0; b < LW; ++b) lrow[i][b] = act ? M[(long)r * ldm + p0 + b] : mk<T>(0, 0);
+          for (int b = 0; b < LW; ++b) lrow[i][b] = mk<T>(0, 0);
+        }
       }
       if (owner) {
 #pragma unroll
@@ -256,9 +289,13 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
     for (int i = 0; i < RPT; ++i) {
       const int r = c0 + t + i * NT;
       if (r < N) {
+        if (w == LW) {
+          RowSlice<T, LW>::store(M + (long)r * ldm + j0, reg[i]);
+        } else {
 #pragma unroll
-        for (int j = 0; j < LW; ++j)
-          if (j < w) M[(long)r * ldm + j0 + j] = reg[i][j];
+          for (int j = 0; j < LW; ++j)
+            if (j < w) M[(long)r * ldm + j0 + j] = reg[i][j];
+        }
       }
     }
     // ---- unit-lower inverse of this leaf's diagonal block (for the later leaves of the panel)
