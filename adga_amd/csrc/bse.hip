@@ -128,8 +128,42 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
   __syncthreads();
   const z64* C = ch == 0 ? c_d : c_m;
   const int dl = t % DB, rl = t / DB;
-  if (dl >= nd) return;
   const int d = d0 + dl;
+  if constexpr (NN <= 3) {
+    // outputs go through an LDS tile so that each row segment (nd*n2 complex) is written by
+    // consecutive lanes
+    constexpr int RS = 256 / DB, W = DB * N2;
+    __shared__ cplx<T> tile[RS][W + 1];
+    for (int r0 = 0; r0 < N; r0 += RS) {
+      const int r = r0 + rl;
+      const bool act = dl < nd && r < N;
+      z64 cc[N2];
+      const z64* crow = C + (long)(act ? r : 0) * N + (act ? d : 0) * N2;
+#pragma unroll
+      for (int i = 0; i < N2; ++i) cc[i] = crow[i];
+      for (int pb = 0; pb < np; ++pb) {
+        if (act) {
+          const z64* sd = sS[pb] + dl * N4;
+#pragma unroll 1
+          for (int sq = 0; sq < N2; ++sq) {
+            z64 acc = mk(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
+            tile[rl][dl * N2 + sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
+          }
+        }
+        __syncthreads();
+        cplx<T>* mb = M + ((long)(2 * (b0 + pb) + ch) * N + r0) * ldm + d0 * N2;
+        for (int e = t; e < RS * W; e += 256) {
+          const int row = e / W, col = e % W;
+          if (r0 + row < N && col < nd * N2) mb[(long)row * ldm + col] = tile[row][col];
+        }
+        __syncthreads();
+      }
+    }
+    return;
+  }
+  if (dl >= nd) return;
   for (int r = rl; r < N; r += 256 / DB) {
     z64 cc[N2];
     const z64* crow = C + (long)r * N + d * N2;
