@@ -441,6 +441,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
     const int j = e / nsg, s = e % nsg;
     x[j * ns + s] = ccast<T>(Rm[(long)perm[j] * nrhs + s]);
   }
+  for (int e = N * ns + t; e < (N + P) * ns; e += NT) x[e] = mk<T>(0, 0);
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {  // 0: forward with unit-lower L, 1: backward with U
     for (int q = 0; q < nb; ++q) {
@@ -525,10 +526,19 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
         if (e < P * nsg) {
           const int r = e / nsg, s = e % nsg;
           if (r0 + r < r1) {
+            // the whole row of D^-1 (zero-padded to P) in one round of loads
             const C* drow = Di + r * P;
             const int c0 = pass == 0 ? 0 : r, c1 = pass == 0 ? r + 1 : r1 - r0;
+            C dv[P];
+#pragma unroll
+            for (int c = 0; c < P; ++c) dv[c] = drow[c];
             C a = mk<T>(0, 0);
-            for (int c = c0; c < c1; ++c) a = cfma(drow[c], x[(r0 + c) * ns + s], a);
+#pragma unroll
+            for (int c = 0; c < P; ++c) {
+              const bool ok = c >= c0 && c < c1;
+              const C xv = x[(r0 + c) * ns + s];  // rows [N, N + P) are zero padding
+              a = cfma(ok ? dv[c] : mk<T>(0, 0), xv, a);
+            }
             o[h] = a;
           }
         }
@@ -756,7 +766,7 @@ LuTimer::~LuTimer() {
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s) {
-  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / N;
+  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / (N + P);  // P zero rows of padding
   int ns = std::min(std::min(nrhs, cap), 16);
   if (ns < 1) return false;
   const int ng = (nrhs + ns - 1) / ns;
