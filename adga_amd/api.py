@@ -42,7 +42,7 @@ def _dptr(a):
 
 
 class Context:
-    def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None, bubble=None, eom=None):
+    def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None, bubble=None, eom=None, lu_streams=None):
         wl = inp.wl
         self.n = wl.n
         self.n2 = wl.n * wl.n
@@ -65,7 +65,7 @@ class Context:
         self.h = h
         self._keep = None
         for key, v in (("batch", batch), ("refine_tol", refine_tol), ("refine_maxit", refine_maxit),
-                       ("bubble", bubble), ("eom", eom)):
+                       ("bubble", bubble), ("eom", eom), ("lu_streams", lu_streams)):
             if v is not None:
                 self._check(lib.adga_set_option(self.h, key.encode(), float(v)))
 

@@ -32,7 +32,8 @@ struct LuTimer {
 };
 template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s, LuTimer* timer = nullptr);
+               int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
+               hipEvent_t lag = nullptr, hipEvent_t join = nullptr);
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s);

@@ -21,6 +21,9 @@ using adga::z64;
 struct adga_ctx {
   Geom g{};
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
+  hipEvent_t ev_lag = nullptr, ev_join = nullptr;
+  int lu_split = 1;                        // option "lu_streams": 2 splits each LU batch over two streams
   std::string err;
   int refine_maxit = 12;
   double refine_tol = 1e-12;
@@ -228,6 +231,9 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     for (int i = 0; i < g.nkp_eom; ++i)
       if (k_data_eom[i] < 1 || k_data_eom[i] > g.nkp) throw HipError{hipErrorInvalidValue, "k_data_eom out of range"};
     CK(hipStreamCreate(&c->stream));
+    CK(hipStreamCreate(&c->stream2));
+    CK(hipEventCreateWithFlags(&c->ev_lag, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     hipStream_t s = c->stream;
     dalloc(&c->hk, (size_t)g.n2 * g.nkp);
     dalloc(&c->siw, (size_t)2 * g.iwmax * g.n);
@@ -280,6 +286,11 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
   if (!strcmp(key, "refine_tol")) {
     if (!(v > 0 && v < 1)) return fail(c, ADGA_E_ARG, "refine_tol must be in (0, 1)");
     c->refine_tol = v;
+    return ADGA_OK;
+  }
+  if (!strcmp(key, "lu_streams")) {
+    if (!(v == 1 || v == 2)) return fail(c, ADGA_E_ARG, "lu_streams must be 1 or 2");
+    c->lu_split = (int)v;
     return ADGA_OK;
   }
   if (!strcmp(key, "refine_maxit")) {
@@ -393,7 +404,8 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, c->ops_f[0], (z64*)c->M, c->piv,
-                                        (z64*)c->inv, c->info, s, &c->lut);
+                                        (z64*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
+                                        c->ev_lag, c->ev_join);
         CK(hipEventRecord(lu1, s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->piv, (const z64*)c->inv,
@@ -412,7 +424,8 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(lu0, s));
         lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, c->ops_f[1], (c32*)c->M, c->piv,
-                                       (c32*)c->inv, c->info, s, &c->lut);
+                                       (c32*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
+                                       c->ev_lag, c->ev_join);
         CK(hipEventRecord(lu1, s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv,
@@ -561,6 +574,9 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
   dfree(c->sig_nl); dfree(c->sig_dmft);
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
+  if (c->ev_lag) (void)hipEventDestroy(c->ev_lag);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ADGA_OK;

@@ -664,8 +664,8 @@ void gemm_launch(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int
 // and short updates do not pay for empty MFMA blocks; 128x128 tiles (64x64 per wave) for the large
 // Schur updates halve their re-reads of A and B.  complex128 stops at 128x64 (registers).
 template <typename T>
-void gemm_op(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
-             const cplx<T>* inv, long istride, hipStream_t s, LuTimer* timer) {
+void gemm_region(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
+                 const cplx<T>* inv, long istride, hipStream_t s, LuTimer* timer) {
   const int tm = m > 64 ? 128 : m > 32 ? 64 : 32;
   const int tn = n > 64 ? 128 : n > 32 ? 64 : 32;
   // the largest tile shape of this precision is the one the timer brackets with events
@@ -697,46 +697,95 @@ void gemm_op(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0,
   if (timed) timer->mark(s);
 }
 
+// An operation is split into its full large tiles and the remainder strips, each launched with the
+// tile shape that fits it: at N = 1080 the top-level update is 536 x 536, which 128x128 tiles alone
+// would cover with 30% of their MFMAs on padding.  The triangular products (mode 1/2) keep all
+// their rows in one tile (in-place), so only their columns are split.
+template <typename T>
+void gemm_op(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K, int mode,
+             const cplx<T>* inv, long istride, hipStream_t s, LuTimer* timer) {
+  constexpr int BR = 128, BC = sizeof(T) == 4 ? 128 : 64;  // largest row / column tile
+  const int mr = (mode == 0 && m > BR) ? m / BR * BR : m;
+  const int nr = n > BC ? n / BC * BC : n;
+  const int rs[2][2] = {{r0, mr}, {r0 + mr, m - mr}};
+  const int cs[2][2] = {{c0, nr}, {c0 + nr, n - nr}};
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      if (rs[i][1] > 0 && cs[j][1] > 0)
+        gemm_region<T>(nmat, mstride, M, ldm, rs[i][0], rs[i][1], cs[j][0], cs[j][1], k0, K, mode, inv, istride, s,
+                       timer);
+}
+
 }  // namespace
 
 // The factorisation: the host walks the operation list and launches one batched kernel per
 // operation (panels: a workgroup per matrix; GEMM/TRSM: a workgroup per output tile per matrix).
+namespace {
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s, LuTimer* timer) {
-  const int rpt = round_rpt((N + NT - 1) / NT);
-  if (rpt < 0 || N > PERM_MAX_N || nmat < 1) return false;
+bool lu_op(int nmat, int N, int W, int ldm, int rpt, const int* op, cplx<T>* M, int* piv, cplx<T>* inv, int* info,
+           hipStream_t s, LuTimer* timer) {
   const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
-  hipLaunchKernelGGL(k_lu_init, dim3((nmat + 255) / 256), dim3(256), 0, s, nmat, info);
-  for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
-    const int* op = &ops[o];
-    if (op[0] == OP_PANEL) {
-      switch (rpt) {
+  if (op[0] == OP_PANEL) {
+    switch (rpt) {
 #define ADGA_P(R)                                                                                             \
   case R:                                                                                                     \
     hipLaunchKernelGGL((k_panel<T, R, LeafW<T, R>::value>), dim3(nmat), dim3(NT), 0, s, N, W, ldm, mstride, op[1], \
                        op[2], M, piv, inv, istride, info);                                                    \
-    break;
-        ADGA_P(1) ADGA_P(2) ADGA_P(3) ADGA_P(4) ADGA_P(5) ADGA_P(6) ADGA_P(8) ADGA_P(12) ADGA_P(16)
+    return true;
+      ADGA_P(1) ADGA_P(2) ADGA_P(3) ADGA_P(4) ADGA_P(5) ADGA_P(6) ADGA_P(8) ADGA_P(12) ADGA_P(16)
 #undef ADGA_P
-        default: return false;
-      }
-    } else if (op[0] == OP_TRSM_L || op[0] == OP_TRSM_U) {
-      const int K = op[2] - op[1];
-      gemm_op<T>(nmat, mstride, M, ldm, 0, K, op[3], op[4] - op[3], op[1], K, op[0] == OP_TRSM_L ? 1 : 2, inv,
-                 istride, s, nullptr);
-    } else {
-      gemm_op<T>(nmat, mstride, M, ldm, op[1], op[2] - op[1], op[3], op[4] - op[3], op[5], op[6] - op[5], 0, inv,
-                 istride, s, timer);
+      default: return false;
+    }
+  } else if (op[0] == OP_TRSM_L || op[0] == OP_TRSM_U) {
+    const int K = op[2] - op[1];
+    gemm_op<T>(nmat, mstride, M, ldm, 0, K, op[3], op[4] - op[3], op[1], K, op[0] == OP_TRSM_L ? 1 : 2, inv, istride,
+               s, nullptr);
+  } else {
+    gemm_op<T>(nmat, mstride, M, ldm, op[1], op[2] - op[1], op[3], op[4] - op[3], op[5], op[6] - op[5], 0, inv,
+               istride, s, timer);
+  }
+  return true;
+}
+}  // namespace
+
+// With a second stream the batch is split in two halves, the second one operation behind the
+// first on its own stream: while one half runs a latency-bound panel the other's Schur update can
+// use the matrix cores.  `lag` and `join` are events the caller owns.
+template <typename T>
+bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
+               int* info, hipStream_t s, LuTimer* timer, hipStream_t s2, hipEvent_t lag, hipEvent_t join) {
+  const int rpt = round_rpt((N + NT - 1) / NT);
+  if (rpt < 0 || N > PERM_MAX_N || nmat < 1) return false;
+  const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
+  const bool split = s2 && lag && join && nmat >= 16;
+  const int na = split ? nmat / 2 : nmat, nb = nmat - na;
+  cplx<T>* Mb = M + (long)na * mstride;
+  cplx<T>* ib = inv + (long)na * istride;
+  int* pb = piv + (long)na * N;
+  int* fb = info + na;
+  hipLaunchKernelGGL(k_lu_init, dim3((nmat + 255) / 256), dim3(256), 0, s, nmat, info);
+  if (split) {
+    if (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess) return false;
+  }
+  for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
+    if (!lu_op<T>(na, N, W, ldm, rpt, &ops[o], M, piv, inv, info, s, timer)) return false;
+    if (split) {
+      if (o == 0 && (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess))
+        return false;
+      if (!lu_op<T>(nb, N, W, ldm, rpt, &ops[o], Mb, pb, ib, fb, s2, nullptr)) return false;
     }
   }
-  hipLaunchKernelGGL(k_lu_perm, dim3(nmat), dim3(NT), 0, s, N, piv);
+  hipLaunchKernelGGL(k_lu_perm, dim3(na), dim3(NT), 0, s, N, piv);
+  if (split) {
+    hipLaunchKernelGGL(k_lu_perm, dim3(nb), dim3(NT), 0, s2, N, pb);
+    if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return false;
+  }
   return true;
 }
 template bool launch_lu<float>(int, int, int, int, const std::vector<int>&, c32*, int*, c32*, int*, hipStream_t,
-                               LuTimer*);
+                               LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
 template bool launch_lu<double>(int, int, int, int, const std::vector<int>&, z64*, int*, z64*, int*, hipStream_t,
-                                LuTimer*);
+                                LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
 
 void LuTimer::mark(hipStream_t s) {
   if (used == ev.size()) {
