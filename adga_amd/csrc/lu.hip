@@ -334,37 +334,73 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
     }
     __syncthreads();
   }
-  // ---- inverses of the panel's L11 (unit lower) and U11 (upper) into the side buffer
+  // ---- L11^-1 (unit lower) into the side buffer, by block recurrence on the leaves' inverses:
+  // X_ii = Li[i], X_ij = -Li[i] sum_{j <= k < i} L_ik X_kj, one block diagonal at a time (U11^-1 is
+  // formed after the factorisation by k_lu_uinv, off this critical path)
   const int w = c1 - c0;
-  for (int e = t; e < w * w; e += NT) {
-    int a = e / w, b = e % w;
-    L.u.v.Lb[a][b] = M[(long)(c0 + a) * ldm + c0 + b];
+  const int nlf = (w + LW - 1) / LW;
+  for (int e = t; e < P * P; e += NT) {
+    const int a = e / P, b = e % P, bi = a / LW, bj = b / LW;
+    L.u.v.Lb[a][b] = (a < w && b < w) ? M[(long)(c0 + a) * ldm + c0 + b] : mk<T>(0, 0);
+    L.u.v.Xb[0][a][b] = (bi == bj && bi < nlf) ? L.Li[bi][a % LW][b % LW] : mk<T>(0, 0);
   }
   __syncthreads();
-  if (t < w) {  // column t of L11^-1, from the top
-    for (int a = 0; a < w; ++a) {
-      C y = mk<T>(a == t ? 1 : 0, 0);
-      if (a > t)
-        for (int b = t; b < a; ++b) y = cfms(L.u.v.Lb[a][b], L.u.v.Xb[0][b][t], y);
-      L.u.v.Xb[0][a][t] = (a >= t) ? y : mk<T>(0, 0);
+  for (int dd = 1; dd < nlf; ++dd) {
+    const int nb = nlf - dd;
+    for (int e = t; e < nb * LW * LW; e += NT) {
+      const int j = e / (LW * LW), i = j + dd, a = (e / LW) % LW, bb = e % LW;
+      C acc = mk<T>(0, 0);
+      for (int k = j * LW; k < i * LW; ++k) acc = cfma(L.u.v.Lb[i * LW + a][k], L.u.v.Xb[0][k][j * LW + bb], acc);
+      L.u.v.Xb[1][i * LW + a][j * LW + bb] = acc;
     }
-  } else if (t >= 64 && t < 64 + w) {  // column c of U11^-1, from the bottom
-    const int c = t - 64;
+    __syncthreads();
+    for (int e = t; e < nb * LW * LW; e += NT) {
+      const int j = e / (LW * LW), i = j + dd, a = (e / LW) % LW, bb = e % LW;
+      C acc = mk<T>(0, 0);
+#pragma unroll
+      for (int c = 0; c < LW; ++c) acc = cfma(L.Li[i][a][c], L.u.v.Xb[1][i * LW + c][j * LW + bb], acc);
+      L.u.v.Xb[0][i * LW + a][j * LW + bb] = mk<T>(-acc.re, -acc.im);
+    }
+    __syncthreads();
+  }
+  C* dst = inv + (long)(c0 / P) * 2 * P * P;
+  for (int e = t; e < P * P; e += NT) {
+    const int a = e / P, b = e % P;
+    dst[e] = (a < w && b < w) ? L.u.v.Xb[0][a][b] : mk<T>(0, 0);
+  }
+}
+
+// U11^-1 of every panel of every matrix (grid: panels x matrices), after the factorisation: column c
+// of the inverse by back substitution, one lane per column.
+template <typename T>
+__global__ void __launch_bounds__(64) k_lu_uinv(int N, int ldm, long mstride, long istride,
+                                                const cplx<T>* __restrict__ Mall, cplx<T>* __restrict__ invall) {
+  using C = cplx<T>;
+  __shared__ C Ub[P][P + 1], Xu[P][P + 1];
+  const int t = threadIdx.x, c0 = blockIdx.x * P, w = min(P, N - c0);
+  const C* M = Mall + (long)blockIdx.y * mstride;
+  for (int e = t; e < P * P; e += 64) {
+    const int a = e / P, b = e % P;
+    Ub[a][b] = (a < w && b < w) ? M[(long)(c0 + a) * ldm + c0 + b] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  if (t < w) {
+    const int c = t;
     for (int a = w - 1; a >= 0; --a) {
       C y = mk<T>(0, 0);
       if (a <= c) {
         y = mk<T>(a == c ? 1 : 0, 0);
-        for (int b = a + 1; b <= c; ++b) y = cfms(L.u.v.Lb[a][b], L.u.v.Xb[1][b][c], y);
-        y = cdiv(y, L.u.v.Lb[a][a]);
+        for (int b = a + 1; b <= c; ++b) y = cfms(Ub[a][b], Xu[b][c], y);
+        y = cdiv(y, Ub[a][a]);
       }
-      L.u.v.Xb[1][a][c] = y;
+      Xu[a][c] = y;
     }
   }
   __syncthreads();
-  C* dst = inv + (long)(c0 / P) * 2 * P * P;
-  for (int e = t; e < 2 * P * P; e += NT) {
-    int s = e / (P * P), a = (e / P) % P, b = e % P;
-    dst[e] = (a < w && b < w) ? L.u.v.Xb[s][a][b] : mk<T>(0, 0);
+  C* dst = invall + (long)blockIdx.y * istride + (long)blockIdx.x * 2 * P * P + P * P;
+  for (int e = t; e < P * P; e += 64) {
+    const int a = e / P, b = e % P;
+    dst[e] = (a < w && b < w) ? Xu[a][b] : mk<T>(0, 0);
   }
 }
 
@@ -767,7 +803,15 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
   if (split) {
     if (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess) return false;
   }
+  bool uinv = false;
+  auto launch_uinv = [&]() {
+    const int np = (N + P - 1) / P;
+    hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, na), dim3(64), 0, s, N, ldm, mstride, istride, M, inv);
+    if (split) hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, nb), dim3(64), 0, s2, N, ldm, mstride, istride, Mb, ib);
+    uinv = true;
+  };
   for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
+    if (ops[o] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
     if (!lu_op<T>(na, N, W, ldm, rpt, &ops[o], M, piv, inv, info, s, timer)) return false;
     if (split) {
       if (o == 0 && (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess))
@@ -775,6 +819,7 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
       if (!lu_op<T>(nb, N, W, ldm, rpt, &ops[o], Mb, pb, ib, fb, s2, nullptr)) return false;
     }
   }
+  if (!uinv) launch_uinv();
   hipLaunchKernelGGL(k_lu_perm, dim3(na), dim3(NT), 0, s, N, piv);
   if (split) {
     hipLaunchKernelGGL(k_lu_perm, dim3(nb), dim3(NT), 0, s2, N, pb);
