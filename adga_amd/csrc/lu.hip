@@ -464,7 +464,9 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
                                                 const int* __restrict__ permall, const cplx<T>* __restrict__ invall,
                                                 const z64* __restrict__ R, z64* __restrict__ Y, int accumulate) {
   using C = cplx<T>;
-  __shared__ C x[SOLVE_LDS_BYTES / sizeof(C)];  // x[j * ns + s]
+  // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
+  // columns without predicates
+  __shared__ C x[SOLVE_LDS_BYTES / sizeof(C)];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
   const int m = blockIdx.x, s0 = blockIdx.y * ns, nsg = min(ns, nrhs - s0);
   const C* M = Mall + (long)m * N * ldm;
@@ -473,12 +475,12 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
   const C* inv = invall + (long)m * nb * 2 * P * P;
   const z64* Rm = R + (long)m * N * nrhs + s0;
   z64* Ym = Y + (long)m * N * nrhs + s0;
-  for (int e = t; e < N * nsg; e += NT) {
-    const int j = e / nsg, s = e % nsg;
-    x[j * ns + s] = ccast<T>(Rm[(long)perm[j] * nrhs + s]);
+  for (int e = t; e < (N + P) * CG; e += NT) {
+    const int j = e / CG, s = e % CG;
+    x[e] = (j < N && s < nsg) ? ccast<T>(Rm[(long)perm[j] * nrhs + s]) : mk<T>(0, 0);
   }
-  for (int e = N * ns + t; e < (N + P) * ns; e += NT) x[e] = mk<T>(0, 0);
   __syncthreads();
+  constexpr int SU = 8, G = 16 * SU;  // loads per row per group; k covered by one group of 16 lanes
   for (int pass = 0; pass < 2; ++pass) {  // 0: forward with unit-lower L, 1: backward with U
     for (int q = 0; q < nb; ++q) {
       const int ib = pass == 0 ? q : nb - 1 - q;
@@ -491,46 +493,52 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
         C acc0[CG], acc1[CG];
 #pragma unroll
         for (int s = 0; s < CG; ++s) acc0[s] = acc1[s] = mk<T>(0, 0);
-        // software pipeline: the next group of SU loads per row is in flight while this one is used
-        constexpr int SU = 8;
         C pa0[SU], pa1[SU], qa0[SU], qa1[SU];
         auto load = [&](C* d0, C* d1, int kb) {
 #pragma unroll
           for (int u = 0; u < SU; ++u) {
-            const int kk = kb + 16 * u;
-            const bool v = kk < k1;
-            d0[u] = v ? La[kk] : mk<T>(0, 0);
-            d1[u] = v ? Lb[kk] : mk<T>(0, 0);
+            d0[u] = La[kb + 16 * u];
+            d1[u] = Lb[kb + 16 * u];
           }
         };
         auto use = [&](const C* d0, const C* d1, int kb) {
 #pragma unroll
           for (int u = 0; u < SU; ++u) {
-            const int kk = kb + 16 * u;
-            if (kk < k1) {
-              const C* xr = x + kk * ns;
+            const C* xr = x + (kb + 16 * u) * CG;
 #pragma unroll
-              for (int s = 0; s < CG; ++s)
-                if (s < nsg) {
-                  const C xv = xr[s];
-                  acc0[s] = cfma(d0[u], xv, acc0[s]);
-                  acc1[s] = cfma(d1[u], xv, acc1[s]);
-                }
+            for (int s = 0; s < CG; ++s) {
+              const C xv = xr[s];
+              acc0[s] = cfma(d0[u], xv, acc0[s]);
+              acc1[s] = cfma(d1[u], xv, acc1[s]);
             }
           }
         };
+        // full groups (every lane's SU columns inside [k0, k1)): software-pipelined, no predicates
+        const int nfull = (k1 - k0) / G;
         int k = k0 + ks;
-        load(pa0, pa1, k);
-        while (k < k1) {
-          int kn = k + 16 * SU;
-          if (kn < k1) load(qa0, qa1, kn);
-          use(pa0, pa1, k);
-          k = kn;
-          if (k >= k1) break;
-          kn = k + 16 * SU;
-          if (kn < k1) load(pa0, pa1, kn);
-          use(qa0, qa1, k);
-          k = kn;
+        if (nfull > 0) {
+          load(pa0, pa1, k);
+          for (int g = 0; g < nfull; g += 2) {
+            if (g + 1 < nfull) load(qa0, qa1, k + G);
+            use(pa0, pa1, k);
+            if (g + 1 >= nfull) {
+              k += G;
+              break;
+            }
+            if (g + 2 < nfull) load(pa0, pa1, k + 2 * G);
+            use(qa0, qa1, k + G);
+            k += 2 * G;
+          }
+        }
+        for (; k < k1; k += 16) {  // the rest: fewer than G columns
+          const C a0 = La[k], a1 = Lb[k];
+          const C* xr = x + k * CG;
+#pragma unroll
+          for (int s = 0; s < CG; ++s) {
+            const C xv = xr[s];
+            acc0[s] = cfma(a0, xv, acc0[s]);
+            acc1[s] = cfma(a1, xv, acc1[s]);
+          }
         }
 #pragma unroll
         for (int s = 0; s < CG; ++s) {
@@ -544,11 +552,10 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
         }
         if (ks == 0) {
 #pragma unroll
-          for (int s = 0; s < CG; ++s)
-            if (s < nsg) {
-              if (ra < r1) x[ra * ns + s] = csub(x[ra * ns + s], acc0[s]);
-              if (rb < r1) x[rb * ns + s] = csub(x[rb * ns + s], acc1[s]);
-            }
+          for (int s = 0; s < CG; ++s) {
+            if (ra < r1) x[ra * CG + s] = csub(x[ra * CG + s], acc0[s]);
+            if (rb < r1) x[rb * CG + s] = csub(x[rb * CG + s], acc1[s]);
+          }
         }
       }
       __syncthreads();
@@ -559,8 +566,8 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
       for (int h = 0; h < 2; ++h) {
         const int e = t + h * NT;
         o[h] = mk<T>(0, 0);
-        if (e < P * nsg) {
-          const int r = e / nsg, s = e % nsg;
+        if (e < P * CG) {
+          const int r = e / CG, s = e % CG;
           if (r0 + r < r1) {
             // the whole row of D^-1 (zero-padded to P) in one round of loads
             const C* drow = Di + r * P;
@@ -568,14 +575,13 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
             C dv[P];
 #pragma unroll
             for (int c = 0; c < P; ++c) dv[c] = drow[c];
-            C a = mk<T>(0, 0);
+            C acc = mk<T>(0, 0);
 #pragma unroll
             for (int c = 0; c < P; ++c) {
               const bool ok = c >= c0 && c < c1;
-              const C xv = x[(r0 + c) * ns + s];  // rows [N, N + P) are zero padding
-              a = cfma(ok ? dv[c] : mk<T>(0, 0), xv, a);
+              acc = cfma(ok ? dv[c] : mk<T>(0, 0), x[(r0 + c) * CG + s], acc);
             }
-            o[h] = a;
+            o[h] = acc;
           }
         }
       }
@@ -583,9 +589,9 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int e = t + h * NT;
-        if (e < P * nsg) {
-          const int r = e / nsg, s = e % nsg;
-          if (r0 + r < r1) x[(r0 + r) * ns + s] = o[h];
+        if (e < P * CG) {
+          const int r = e / CG;
+          if (r0 + r < r1) x[r0 * CG + e] = o[h];
         }
       }
       __syncthreads();
@@ -593,7 +599,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
   }
   for (int e = t; e < N * nsg; e += NT) {
     const int j = e / nsg, s = e % nsg;
-    const C v = x[j * ns + s];
+    const C v = x[j * CG + s];
     z64 y = mk((double)v.re, (double)v.im);
     if (accumulate) y = cadd(Ym[(long)j * nrhs + s], y);
     Ym[(long)j * nrhs + s] = y;
@@ -856,18 +862,24 @@ LuTimer::~LuTimer() {
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
 }
 
-// columns per group: as many as the LDS holds (at most 16); false when not even one column fits
+// columns per group: one group when a solution of nrhs columns (padded to the kernel's column count)
+// fits the LDS, else groups as wide as fit; false when not even one column fits
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s) {
-  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / (N + P);  // P zero rows of padding
-  int ns = std::min(std::min(nrhs, cap), 16);
-  if (ns < 1) return false;
-  const int ng = (nrhs + ns - 1) / ns;
-  ns = (nrhs + ng - 1) / ng;  // balance the groups
+  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / (N + P);  // columns that fit, with P zero rows
+  static const int cgs[] = {1, 2, 4, 9, 16};
+  int cg = 0;
+  for (int c : cgs)
+    if (c <= cap && c >= nrhs) { cg = c; break; }
+  if (!cg)
+    for (int c : cgs)
+      if (c <= cap) cg = c;
+  if (!cg) return false;
+  const int ns = std::min(nrhs, cg), ng = (nrhs + ns - 1) / ns;
   dim3 grid(nmat, ng);
 #define ADGA_SOLVE_CASE(CGV)                                                                             \
-  if (ns <= CGV) {                                                                                       \
+  if (cg == CGV) {                                                                                       \
     hipLaunchKernelGGL((k_lu_solve<T, CGV>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
                        accumulate ? 1 : 0);                                                              \
     return true;                                                                                         \
