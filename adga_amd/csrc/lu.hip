@@ -60,8 +60,69 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
   }
 }
 
-// Panel LDS
-template <typename T, int LW>
+// component-wise select of two complex values (a select of whole structs can turn into a select of
+// their addresses, which forces register arrays into scratch memory)
+template <typename T>
+__device__ __forceinline__ cplx<T> csel(bool c, cplx<T> a, cplx<T> b) {
+  return mk<T>(c ? a.re : b.re, c ? a.im : b.im);
+}
+
+// Lane moves inside 16-lane rows (DPP row_ror) and reductions over a row: every lane of the row
+// ends with the row's result.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename T>
+__device__ __forceinline__ T row_max(T v) {  // no NaN among the values
+  T o = dpp_v<0x121>(v);
+  v = o > v ? o : v;
+  o = dpp_v<0x122>(v);
+  v = o > v ? o : v;
+  o = dpp_v<0x124>(v);
+  v = o > v ? o : v;
+  o = dpp_v<0x128>(v);
+  return o > v ? o : v;
+}
+__device__ __forceinline__ int row_min(int v) {
+  v = min(v, dpp_i<0x121>(v));
+  v = min(v, dpp_i<0x122>(v));
+  v = min(v, dpp_i<0x124>(v));
+  return min(v, dpp_i<0x128>(v));
+}
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int l);
+template <>
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <>
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// the wave's maximum, the same value in every lane
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  v = row_max(v);
+  const T a = lane_value(v, 0), b = lane_value(v, 16), c = lane_value(v, 32), d = lane_value(v, 48);
+  const T ab = a > b ? a : b, cd = c > d ? c : d;
+  return ab > cd ? ab : cd;
+}
+
+// Panel LDS (NW waves)
+template <typename T, int LW, int NW>
 struct PanelLds {
   struct {
     struct {
@@ -71,9 +132,9 @@ struct PanelLds {
   } u;
   // pivot search, double-buffered by column parity: each wave's best candidate (|re|+|im|, row) with
   // the row's leaf values, and the values of the row the pivot replaces
-  cplx<T> cv[2][NT / 64][LW];
-  T cval[2][NT / 64];
-  int crow[2][NT / 64];
+  cplx<T> cv[2][NW][LW];
+  T cval[2][NW];
+  int crow[2][NW];
   cplx<T> kv[2][LW];
   cplx<T> Xs[2][LW][LW + 1];       // left-looking: [0] raw rows of the current leaf, [1] their U block
   cplx<T> Li[P / LW][LW][LW + 1];  // unit-lower inverses of the panel's leaves
@@ -107,18 +168,23 @@ struct RowSlice {
 };
 
 // ---------------------------------------------------------------- panel factorisation
-// Columns [c0, c1) of rows [c0, N); thread t holds rows c0 + t + i*NT (i < RPT) of the current leaf
-// of LW columns.  Barriers: two per (leaf, earlier leaf) pair of the left-looking updates, one per
-// column for the pivot (each wave publishes its best candidate together with that row's values, so
-// the winner's row needs no second round), two per leaf.
-template <typename T, int RPT, int LW>
+// Columns [c0, c1) of rows [c0, N) with NTP threads; thread t holds rows c0 + t + i*NTP (i < RPT)
+// of the current leaf of LW columns (rows >= N as zeros).  Barriers: two per (leaf, earlier leaf)
+// pair of the left-looking updates, one per column for the pivot (each wave publishes its best
+// candidate together with that row's values, so the winner's row needs no second round), one per
+// leaf.  Row updates are written without branches on the row (a multiplier of zero leaves a row
+// as it is); a slot whose rows are all outside the update for the whole wave is skipped on a
+// wave-uniform test.
+template <typename T, int RPT, int LW, int NTP>
 __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N, int W, int c0, int c1,
-                                         int* __restrict__ piv, cplx<T>* __restrict__ inv, PanelLds<T, LW>& L,
-                                         int& myinfo) {
+                                         int* __restrict__ piv, cplx<T>* __restrict__ inv,
+                                         PanelLds<T, LW, NTP / 64>& L, int& myinfo) {
   using C = cplx<T>;
-  constexpr int NWAVE = NT / 64;
-  constexpr int NOROW = 0x7fffffff;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr int NW = NTP / 64;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int rb = c0 + t;            // this thread's row in slot 0
+  const int wr = c0 + wave * 64;    // the wave's first row in slot 0 (uniform)
   const int nl = (c1 - c0 + LW - 1) / LW;
   int q = 0;  // parity of the pivot-search buffers
   for (int jl = 0; jl < nl; ++jl) {
@@ -127,12 +193,16 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
     C reg[RPT][LW];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
-      int r = c0 + t + i * NT;
+      const int r = rb + i * NTP;
       if (r < N && w == LW) {
         RowSlice<T, LW>::load(M + (long)r * ldm + j0, reg[i]);
       } else {
 #pragma unroll
-        for (int j = 0; j < LW; ++j) reg[i][j] = (r < N && j < w) ? M[(long)r * ldm + j0 + j] : mk<T>(0, 0);
+        for (int j = 0; j < LW; ++j) {
+          C v = mk<T>(0, 0);
+          if (r < N && j < w) v = M[(long)r * ldm + j0 + j];
+          reg[i][j] = v;
+        }
       }
     }
     // ---- left-looking updates from the panel's previous leaves
@@ -140,10 +210,10 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
       const int p0 = c0 + il * LW;
       const int a = t - (p0 - c0);  // this thread owns row p0 + a (slot 0) when 0 <= a < LW
       const bool owner = a >= 0 && a < LW;
-      C lrow[RPT][LW];  // L values of the rows below the earlier leaf, loaded ahead of the barriers
+      C lrow[RPT][LW];  // L values of the rows below the earlier leaf (zero elsewhere), loaded ahead
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
-        const int r = c0 + t + i * NT;
+        const int r = rb + i * NTP;
         if (r >= p0 + LW && r < N) {
           RowSlice<T, LW>::load(M + (long)r * ldm + p0, lrow[i]);
         } else {
@@ -156,138 +226,162 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
         for (int j = 0; j < LW; ++j) L.Xs[0][a][j] = reg[0][j];
       }
       __syncthreads();
-      if (owner) {  // row a of the U block: L_ii^-1 applied to the raw rows
-        C xn[LW];
+      if (t < LW * LW) {  // U block element (a2, jc): L_ii^-1 applied to the raw rows
+        const int a2 = t / LW, jc = t % LW;
+        C acc = mk<T>(0, 0);
 #pragma unroll
-        for (int j = 0; j < LW; ++j) {
-          C acc = mk<T>(0, 0);
-#pragma unroll
-          for (int b = 0; b < LW; ++b)
-            if (b <= a) acc = cfma(L.Li[il][a][b], L.Xs[0][b][j], acc);
-          xn[j] = acc;
-        }
-#pragma unroll
-        for (int j = 0; j < LW; ++j) {
-          reg[0][j] = xn[j];
-          L.Xs[1][a][j] = xn[j];
-        }
+        for (int b = 0; b < LW; ++b) acc = cfma(L.Li[il][a2][b], L.Xs[0][b][jc], acc);
+        L.Xs[1][a2][jc] = acc;
       }
       __syncthreads();
+      if (owner) {
 #pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int r = c0 + t + i * NT;
-        if (r >= p0 + LW && r < N) {
+        for (int j = 0; j < LW; ++j) reg[0][j] = L.Xs[1][a][j];
+      }
 #pragma unroll
-          for (int b = 0; b < LW; ++b)
+      for (int b = 0; b < LW; ++b) {
+        C u[LW];
 #pragma unroll
-            for (int j = 0; j < LW; ++j) reg[i][j] = cfms(lrow[i][b], L.Xs[1][b][j], reg[i][j]);
+        for (int j = 0; j < LW; ++j) u[j] = L.Xs[1][b][j];
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          if (wr + i * NTP + 63 < p0 + LW || wr + i * NTP >= N) continue;
+#pragma unroll
+          for (int j = 0; j < LW; ++j) reg[i][j] = cfms(lrow[i][b], u[j], reg[i][j]);
         }
       }
     }
-    // ---- factor the leaf (rows >= kc are pivot candidates)
+    // ---- factor the leaf (rows >= kc are pivot candidates).  Step j resolves column j's pivot
+    // and multipliers, updates column j + 1, runs column j + 1's pivot search, and only then
+    // updates the columns beyond j + 1 (that update fills the latency of the search's reductions);
+    // the candidate rows are published whole after it.  Step -1 only searches column 0.
 #pragma unroll
-    for (int j = 0; j < LW; ++j) {
-      if (j < w) {
+    for (int j = -1; j < LW; ++j) {
+      if (j >= w) continue;
+      C pr[LW], lm[RPT];
+      if (j >= 0) {
         const int kc = j0 + j;
-        T bv = (T)-1;
-        int br = NOROW, bs = 0;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-          const int r = c0 + t + i * NT;
-          if (r < N && r >= kc) {
-            const T v = cabs1(reg[i][j]);
-            if (v > bv) { bv = v; br = r; bs = i; }
-          }
-        }
-        T wv = bv;
-        int wr = br;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const T ov = __shfl_xor(wv, off);
-          const int orow = __shfl_xor(wr, off);
-          if (ov > wv || (ov == wv && orow < wr)) { wv = ov; wr = orow; }
-        }
-        if (wr != NOROW && br == wr) {  // the lane holding the wave's best row publishes it
-          L.cval[q][wave] = wv;
-          L.crow[q][wave] = wr;
-#pragma unroll
-          for (int i = 0; i < RPT; ++i)
-            if (i == bs) {
-#pragma unroll
-              for (int jj = 0; jj < LW; ++jj) L.cv[q][wave][jj] = reg[i][jj];
-            }
-        } else if (wr == NOROW && lane == 0) {
-          L.cval[q][wave] = (T)-1;
-          L.crow[q][wave] = NOROW;
-        }
-        const int tk = (kc - c0) % NT, sk = (kc - c0) / NT;
-        if (t == tk) {
-#pragma unroll
-          for (int i = 0; i < RPT; ++i)
-            if (i == sk) {
-#pragma unroll
-              for (int jj = 0; jj < LW; ++jj) L.kv[q][jj] = reg[i][jj];
-            }
-        }
         __syncthreads();
-        T pbv = L.cval[q][0];
-        int p = L.crow[q][0], pw = 0;
-#pragma unroll
-        for (int w2 = 1; w2 < NWAVE; ++w2) {
-          const T ov = L.cval[q][w2];
-          const int orow = L.crow[q][w2];
-          if (ov > pbv || (ov == pbv && orow < p)) { pbv = ov; p = orow; pw = w2; }
+        // the workgroup's pivot: the waves' candidates in lanes 0 .. NW-1 (one DPP row); ties in
+        // value go to the smallest row
+        T gv = (T)-1;
+        int grow = 0x7fffffff;
+        if (lane < NW) {
+          gv = L.cval[q][lane];
+          grow = L.crow[q][lane];
         }
+        const T gm = lane_value(row_max(gv), 0);
+        const unsigned long long tied = __ballot(lane < NW && gm >= (T)0 && gv == gm);
+        int gr, pw;
+        if (__popcll(tied) <= 1) {
+          pw = tied ? __ffsll((long long)tied) - 1 : 0;
+          gr = __builtin_amdgcn_readlane(grow, pw);
+        } else {
+          gr = __builtin_amdgcn_readlane(row_min((gm >= (T)0 && gv == gm) ? grow : 0x7fffffff), 0);
+          const unsigned long long pb = __ballot(lane < NW && gm >= (T)0 && gv == gm && grow == gr);
+          pw = __ffsll((long long)pb) - 1;
+        }
+        int p = gm >= (T)0 ? gr : N;
         // no finite candidate (NaN column): keep the diagonal row and flag the matrix
         if (p >= N) {
           p = kc;
           if (myinfo == 0) myinfo = kc + 1;
         }
-        C pr[LW];
+        const C* src = (p == kc) ? &L.kv[q][0] : &L.cv[q][pw][0];
 #pragma unroll
-        for (int jj = 0; jj < LW; ++jj) pr[jj] = (p == kc) ? L.kv[q][jj] : L.cv[q][pw][jj];
+        for (int jj = 0; jj < LW; ++jj) pr[jj] = src[jj];
         if (t == 0) { L.spiv[j] = p; piv[kc] = p; }
-        if (p != kc) {
-          const int tp = (p - c0) % NT, sp = (p - c0) / NT;
-          if (t == tp) {
+        if (p != kc) {  // the interchange, in registers: row p gets the old row kc and vice versa
+          // (selected by row, not by slot index: an index into reg would move it to scratch memory)
+          C kvv[LW];
 #pragma unroll
-            for (int i = 0; i < RPT; ++i)
-              if (i == sp) {
+          for (int jj = 0; jj < LW; ++jj) kvv[jj] = L.kv[q][jj];
 #pragma unroll
-                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = L.kv[q][jj];
-              }
-          }
-          if (t == tk) {
+          for (int i = 0; i < RPT; ++i) {
+            const int r = rb + i * NTP;
+            const bool hp = r == p, hk = r == kc;
 #pragma unroll
-            for (int i = 0; i < RPT; ++i)
-              if (i == sk) {
-#pragma unroll
-                for (int jj = 0; jj < LW; ++jj) reg[i][jj] = pr[jj];
-              }
+            for (int jj = 0; jj < LW; ++jj) reg[i][jj] = csel(hp, kvv[jj], csel(hk, pr[jj], reg[i][jj]));
           }
         }
         const C pv = pr[j];
-        if (pv.re == (T)0 && pv.im == (T)0) {
-          if (myinfo == 0) myinfo = kc + 1;
-        } else {
-          const C rp = crecip(pv);
+        const bool nz = !(pv.re == (T)0 && pv.im == (T)0);
+        if (!nz && myinfo == 0) myinfo = kc + 1;  // zero pivot: no elimination in this column
+        const C rp = crecip(pv);
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const bool below = nz && rb + i * NTP > kc;  // rows >= N hold zeros and stay zero
+          const C l = cmul(reg[i][j], rp);
+          lm[i] = csel(below, l, mk<T>(0, 0));
+          reg[i][j] = csel(below, l, reg[i][j]);
+        }
+        q ^= 1;
+      }
+      const int jn = j + 1;
+      int wrow = -1, wslot = 0, wlane = 0;  // this wave's candidate for column jn
+      if (jn < LW && jn < w) {
+        const int kn = j0 + jn;
+        if (j >= 0) {
+#pragma unroll
+          for (int i = 0; i < RPT; ++i) reg[i][jn] = cfms(lm[i], pr[jn], reg[i][jn]);
+        }
+        // this thread's best candidate for column jn (|re|+|im|, -1 for none; the first slot on ties)
+        T bvv = (T)-1;
+        int bs = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const int r = rb + i * NTP;
+          const T v = cabs1(reg[i][jn]);
+          const T vc = (r >= kn && r < N && v == v) ? v : (T)-1;
+          const bool take = vc > bvv;
+          bvv = take ? vc : bvv;
+          bs = take ? i : bs;
+        }
+        // the wave's best value, and its smallest row (slot first, then lane): uniform
+        const T wm = wave_max(bvv);
+        if (wm >= (T)0) {
 #pragma unroll
           for (int i = 0; i < RPT; ++i) {
-            const int r = c0 + t + i * NT;
-            if (r < N && r > kc) {
-              const C l = cmul(reg[i][j], rp);
-              reg[i][j] = l;
-#pragma unroll
-              for (int jj = j + 1; jj < LW; ++jj) reg[i][jj] = cfms(l, pr[jj], reg[i][jj]);
+            if (wrow < 0) {
+              const unsigned long long b = __ballot(bvv == wm && bs == i);
+              if (b != 0ull) {
+                wlane = __ffsll((long long)b) - 1;
+                wslot = i;
+                wrow = wr + wlane + i * NTP;
+              }
             }
           }
         }
-        q ^= 1;
+        if (lane == 0) {
+          L.cval[q][wave] = wm;
+          L.crow[q][wave] = wrow;
+        }
+      }
+      if (j >= 0) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+#pragma unroll
+          for (int jj = j + 2; jj < LW; ++jj) reg[i][jj] = cfms(lm[i], pr[jj], reg[i][jj]);
+        }
+      }
+      if (jn < LW && jn < w) {  // rows published whole, after the update of all their columns
+        if (wrow >= 0 && lane == wlane) {  // the row's leaf values go with the candidate
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (i == wslot) {
+#pragma unroll
+              for (int jj = 0; jj < LW; ++jj) L.cv[q][wave][jj] = reg[i][jj];
+            }
+        }
+        if (t == j0 + jn - c0) {  // the diagonal row is slot 0 of thread kn - c0 (kn - c0 < P)
+#pragma unroll
+          for (int jj = 0; jj < LW; ++jj) L.kv[q][jj] = reg[0][jj];
+        }
       }
     }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
-      const int r = c0 + t + i * NT;
+      const int r = rb + i * NTP;
       if (r < N) {
         if (w == LW) {
           RowSlice<T, LW>::store(M + (long)r * ldm + j0, reg[i]);
@@ -307,19 +401,22 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
       }
     }
     __syncthreads();
-    if (t < LW) {  // column t of the inverse: y_t = 1, y_a = -sum_{t<=b<a} L_ab y_b
+    if (t < LW) {  // column t of the inverse: y_a = [a == t] - sum_{b < a} L_ab y_b (zero above t)
+      C y[LW];
+#pragma unroll
       for (int a = 0; a < LW; ++a) {
-        C y = mk<T>(a == t ? 1 : 0, 0);
-        if (a > t && a < w)
-          for (int b = t; b < a; ++b) y = cfms(L.Xs[0][a][b], L.Li[jl][b][t], y);
-        L.Li[jl][a][t] = (a < w && t < w) ? y : mk<T>(0, 0);
+        C acc = mk<T>(a == t ? 1 : 0, 0);
+#pragma unroll
+        for (int b = 0; b < a; ++b) acc = cfms(L.Xs[0][a][b], y[b], acc);
+        y[a] = csel(a < w && t < w, acc, mk<T>(0, 0));
+        L.Li[jl][a][t] = y[a];
       }
     }
     // ---- this leaf's interchanges on every other column of the rows
     bool any = false;
     for (int j = 0; j < w; ++j) any |= (L.spiv[j] != j0 + j);
     if (any) {
-      for (int c = t; c < W; c += NT) {
+      for (int c = t; c < W; c += NTP) {
         if (c >= j0 && c < j0 + w) continue;
         for (int j = 0; j < w; ++j) {
           int p = L.spiv[j];
@@ -339,7 +436,7 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
   // formed after the factorisation by k_lu_uinv, off this critical path)
   const int w = c1 - c0;
   const int nlf = (w + LW - 1) / LW;
-  for (int e = t; e < P * P; e += NT) {
+  for (int e = t; e < P * P; e += NTP) {
     const int a = e / P, b = e % P, bi = a / LW, bj = b / LW;
     L.u.v.Lb[a][b] = (a < w && b < w) ? M[(long)(c0 + a) * ldm + c0 + b] : mk<T>(0, 0);
     L.u.v.Xb[0][a][b] = (bi == bj && bi < nlf) ? L.Li[bi][a % LW][b % LW] : mk<T>(0, 0);
@@ -347,14 +444,14 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
   __syncthreads();
   for (int dd = 1; dd < nlf; ++dd) {
     const int nb = nlf - dd;
-    for (int e = t; e < nb * LW * LW; e += NT) {
+    for (int e = t; e < nb * LW * LW; e += NTP) {
       const int j = e / (LW * LW), i = j + dd, a = (e / LW) % LW, bb = e % LW;
       C acc = mk<T>(0, 0);
       for (int k = j * LW; k < i * LW; ++k) acc = cfma(L.u.v.Lb[i * LW + a][k], L.u.v.Xb[0][k][j * LW + bb], acc);
       L.u.v.Xb[1][i * LW + a][j * LW + bb] = acc;
     }
     __syncthreads();
-    for (int e = t; e < nb * LW * LW; e += NT) {
+    for (int e = t; e < nb * LW * LW; e += NTP) {
       const int j = e / (LW * LW), i = j + dd, a = (e / LW) % LW, bb = e % LW;
       C acc = mk<T>(0, 0);
 #pragma unroll
@@ -364,7 +461,7 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
     __syncthreads();
   }
   C* dst = inv + (long)(c0 / P) * 2 * P * P;
-  for (int e = t; e < P * P; e += NT) {
+  for (int e = t; e < P * P; e += NTP) {
     const int a = e / P, b = e % P;
     dst[e] = (a < w && b < w) ? L.u.v.Xb[0][a][b] : mk<T>(0, 0);
   }
@@ -405,17 +502,17 @@ __global__ void __launch_bounds__(64) k_lu_uinv(int N, int ldm, long mstride, lo
 }
 
 // ---------------------------------------------------------------- panel, pivots
-// PANEL(c0, c1) of every matrix of the batch, one workgroup per matrix.  info[m] keeps the first
-// zero pivot (1-based, LAPACK's info), or stays 0.
-template <typename T, int RPT, int LW>
-__global__ void __launch_bounds__(NT) k_panel(int N, int W, int ldm, long mstride, int c0, int c1,
-                                              cplx<T>* __restrict__ Mall, int* __restrict__ pivall,
-                                              cplx<T>* __restrict__ invall, long istride, int* __restrict__ info) {
-  __shared__ PanelLds<T, LW> L;
+// PANEL(c0, c1) of every matrix of the batch, one workgroup of NTP threads per matrix.  info[m]
+// keeps the first zero pivot (1-based, LAPACK's info), or stays 0.
+template <typename T, int RPT, int LW, int NTP>
+__global__ void __launch_bounds__(NTP) k_panel(int N, int W, int ldm, long mstride, int c0, int c1,
+                                               cplx<T>* __restrict__ Mall, int* __restrict__ pivall,
+                                               cplx<T>* __restrict__ invall, long istride, int* __restrict__ info) {
+  __shared__ PanelLds<T, LW, NTP / 64> L;
   const int mat = blockIdx.x;
   int myinfo = 0;
-  wg_panel<T, RPT, LW>(Mall + (long)mat * mstride, ldm, N, W, c0, c1, pivall + (long)mat * N,
-                       invall + (long)mat * istride, L, myinfo);
+  wg_panel<T, RPT, LW, NTP>(Mall + (long)mat * mstride, ldm, N, W, c0, c1, pivall + (long)mat * N,
+                            invall + (long)mat * istride, L, myinfo);
   if (threadIdx.x == 0 && myinfo != 0 && info[mat] == 0) info[mat] = myinfo;
 }
 
@@ -673,10 +770,16 @@ struct LeafW {
   static constexpr int value = (sizeof(T) == 4 && RPT <= 6) ? 8 : 4;
 };
 
-static int round_rpt(int rpt) {
-  static const int opts[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+// panel shape by the panel's row count (rows c0 .. N-1): 256 threads up to 1536 rows, else 512,
+// with the fewest rows per thread (RPT) that cover them, encoded as RPT + 100 for the 512-thread
+// form; -1 beyond 4096 rows.  Fitting RPT to each panel keeps the per-column work of the later,
+// shorter panels proportional to their rows.
+static int panel_cfg(int rows) {
+  if (rows <= 1536) return std::max(1, (rows + 255) / 256);
+  static const int opts[] = {4, 5, 6, 8};
+  const int rpt = (rows + 511) / 512;
   for (int o : opts)
-    if (rpt <= o) return o;
+    if (rpt <= o) return 100 + o;
   return -1;
 }
 
@@ -764,17 +867,18 @@ void gemm_op(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0,
 // operation (panels: a workgroup per matrix; GEMM/TRSM: a workgroup per output tile per matrix).
 namespace {
 template <typename T>
-bool lu_op(int nmat, int N, int W, int ldm, int rpt, const int* op, cplx<T>* M, int* piv, cplx<T>* inv, int* info,
+bool lu_op(int nmat, int N, int W, int ldm, const int* op, cplx<T>* M, int* piv, cplx<T>* inv, int* info,
            hipStream_t s, LuTimer* timer) {
   const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
   if (op[0] == OP_PANEL) {
-    switch (rpt) {
-#define ADGA_P(R)                                                                                             \
-  case R:                                                                                                     \
-    hipLaunchKernelGGL((k_panel<T, R, LeafW<T, R>::value>), dim3(nmat), dim3(NT), 0, s, N, W, ldm, mstride, op[1], \
-                       op[2], M, piv, inv, istride, info);                                                    \
+    switch (panel_cfg(N - op[1])) {
+#define ADGA_P(R, NTP)                                                                                      \
+  case (NTP == 512 ? 100 : 0) + R:                                                                          \
+    hipLaunchKernelGGL((k_panel<T, R, LeafW<T, R>::value, NTP>), dim3(nmat), dim3(NTP), 0, s, N, W, ldm,    \
+                       mstride, op[1], op[2], M, piv, inv, istride, info);                                  \
     return true;
-      ADGA_P(1) ADGA_P(2) ADGA_P(3) ADGA_P(4) ADGA_P(5) ADGA_P(6) ADGA_P(8) ADGA_P(12) ADGA_P(16)
+      ADGA_P(1, 256) ADGA_P(2, 256) ADGA_P(3, 256) ADGA_P(4, 256) ADGA_P(5, 256) ADGA_P(6, 256)
+      ADGA_P(4, 512) ADGA_P(5, 512) ADGA_P(6, 512) ADGA_P(8, 512)
 #undef ADGA_P
       default: return false;
     }
@@ -796,8 +900,7 @@ bool lu_op(int nmat, int N, int W, int ldm, int rpt, const int* op, cplx<T>* M, 
 template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
                int* info, hipStream_t s, LuTimer* timer, hipStream_t s2, hipEvent_t lag, hipEvent_t join) {
-  const int rpt = round_rpt((N + NT - 1) / NT);
-  if (rpt < 0 || N > PERM_MAX_N || nmat < 1) return false;
+  if (panel_cfg(N) < 0 || N > PERM_MAX_N || nmat < 1) return false;
   const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
   const bool split = s2 && lag && join && nmat >= 16;
   const int na = split ? nmat / 2 : nmat, nb = nmat - na;
@@ -818,11 +921,11 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
   };
   for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
     if (ops[o] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
-    if (!lu_op<T>(na, N, W, ldm, rpt, &ops[o], M, piv, inv, info, s, timer)) return false;
+    if (!lu_op<T>(na, N, W, ldm, &ops[o], M, piv, inv, info, s, timer)) return false;
     if (split) {
       if (o == 0 && (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess))
         return false;
-      if (!lu_op<T>(nb, N, W, ldm, rpt, &ops[o], Mb, pb, ib, fb, s2, nullptr)) return false;
+      if (!lu_op<T>(nb, N, W, ldm, &ops[o], Mb, pb, ib, fb, s2, nullptr)) return false;
     }
   }
   if (!uinv) launch_uinv();
