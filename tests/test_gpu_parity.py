@@ -61,7 +61,8 @@ def test_gpu_matches_reference_golden(name, precision, modes):
 def test_gpu_inverse_matrix_matches_numpy():
     api = _api()
     rng = np.random.default_rng(3)
-    for n in (1, 3, 40, 300):
+    # sizes that reach every panel shape: rows <= 256 .. 1536 per 256-thread panel, > 1536 on 512
+    for n in (1, 3, 40, 300, 700, 1300, 1700):
         a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.eye(n)
         inv, ierr = api.inverse_matrix(a)
         assert ierr == 0
