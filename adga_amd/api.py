@@ -51,6 +51,8 @@ class Context:
         self.iwbmax_small = wl.iwbmax_small
         self.nkp_eom = len(inp.k_data_eom)
         self.nqp = len(inp.q_data)
+        self.iwmax = int(inp.iwmax)
+        self.nkp = int(wl.nk[0] * wl.nk[1] * wl.nk[2])
         d = _lib.adga_dims(wl.n, wl.nk[0], wl.nk[1], wl.nk[2], inp.iwmax, wl.iwfmax_small, wl.iwbmax_small,
                            self.nqp, self.nkp_eom, float(inp.beta), float(inp.mu))
         self._keep = [_zf(inp.hk), _zf(inp.siw), _zf(inp.giw), _zf(inp.dc), _zf(inp.u), _zf(inp.u_tilde),
@@ -132,6 +134,41 @@ class Context:
         shf = np.zeros((n, n, self.nkp_eom), dtype=np.complex128, order="F")
         self._check(lib.adga_get_sigma(self.h, _dptr(snl), _dptr(sdm), _dptr(shf)))
         return snl, sdm, shf
+
+    # ----------------------------------------------------------------- one-particle routines
+    def get_giw(self):
+        """giw(-iwmax:iwmax-1, ndim) like get_giw (src/one_particle_quant_module.f90:40-79)."""
+        out = np.zeros((2 * self.iwmax, self.n), dtype=np.complex128, order="F")
+        self._check(lib.adga_get_giw(self.h, _dptr(out)))
+        return out
+
+    def get_nfock(self):
+        """n_fock(nkp, ndim, ndim) like get_nfock (src/one_particle_quant_module.f90:337-361)."""
+        out = np.zeros((self.nkp, self.n, self.n), dtype=np.complex128, order="F")
+        self._check(lib.adga_get_nfock(self.h, _dptr(out)))
+        return out
+
+    def get_sigma_g_loc(self, sigma_sum):
+        """(sigma_loc, gloc) like get_sigma_g_loc (src/one_particle_quant_module.f90:124-180);
+        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp) after add_siw_dmft."""
+        ss = _zf(sigma_sum)
+        if ss.shape != (self.n, self.n, self.nf, self.nkp):
+            raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp})")
+        sloc = np.zeros((self.n, self.n, self.nf), dtype=np.complex128, order="F")
+        gloc = np.zeros((2 * self.iwmax, self.n, self.n), dtype=np.complex128, order="F")
+        self._check(lib.adga_get_sigma_g_loc(self.h, _dptr(ss), _dptr(sloc), _dptr(gloc)))
+        return sloc, gloc
+
+    def get_ndga(self, sigma_sum):
+        """(n_dga_k, n_dga) like get_ndga (src/one_particle_quant_module.f90:363-393);
+        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) after add_siw_dmft."""
+        ss = _zf(sigma_sum)
+        if ss.shape != (self.n, self.n, self.nf, self.nkp_eom):
+            raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp_eom})")
+        ndk = np.zeros((self.nkp_eom, self.n, self.n), dtype=np.complex128, order="F")
+        nd = np.zeros(self.n, dtype=np.complex128)
+        self._check(lib.adga_get_ndga(self.h, _dptr(ss), _dptr(ndk), _dptr(nd)))
+        return ndk, nd
 
     def reset_sigma(self):
         self._check(lib.adga_reset_sigma(self.h))

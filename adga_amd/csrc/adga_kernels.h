@@ -7,6 +7,13 @@
 namespace adga {
 
 void launch_gtable(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* gt, hipStream_t s);
+// one-particle routines (onep.hip); false for an orbital count the kernels do not cover (n > 8)
+bool launch_op_giw(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* giw, hipStream_t s);
+bool launch_op_nfock(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* nfock, hipStream_t s);
+bool launch_op_sigma_g_loc(const Geom& g, const z64* hk, const z64* siw, const z64* dc, const z64* ssum, z64* sloc,
+                           z64* gloc, hipStream_t s);
+bool launch_op_ndga(const Geom& g, const z64* hk, const z64* siw, const z64* dc, const int* keom, const z64* ssum,
+                    z64* ndk, z64* nd, hipStream_t s);
 void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z64* xm, z64* winv, z64* chi0w,
                   z64* cd, z64* cm, z64* gam_d, z64* gam_m, z64* opg_d, z64* opg_m, z64* chiloc_d,
                   z64* chiloc_m, z64* bubloc, hipStream_t s);

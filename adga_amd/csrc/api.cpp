@@ -190,6 +190,23 @@ void ensure_ops(adga_ctx* c, int prec) {
 
 }  // namespace
 
+namespace {
+// device buffer released on every exit path
+struct DevBuf {
+  z64* p = nullptr;
+  explicit DevBuf(size_t count) { dalloc(&p, count); }
+  ~DevBuf() { dfree(p); }
+};
+int op_finish(adga_ctx* c, bool launched, const char* what) {
+  if (!launched) {
+    c->err = std::string(what) + ": orbital count above 8";
+    return ADGA_E_ARG;
+  }
+  CK(hipGetLastError());
+  return ADGA_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom, const double* hk,
@@ -536,6 +553,72 @@ int adga_get_sigma(adga_ctx* c, double* sig_nl, double* sig_dmft, double* sig_hf
     if (sig_dmft) CK(hipMemcpyAsync(sig_dmft, c->sig_dmft, sizeof(z64) * g.n2 * g.nf, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     if (sig_hf) memset(sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom);
+    return ADGA_OK;
+  });
+}
+
+int adga_get_giw(adga_ctx* c, double* giw) {
+  if (!c || !giw) return ADGA_E_ARG;
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    const size_t ne = (size_t)2 * g.iwmax * g.n;
+    DevBuf d(ne);
+    const int rc = op_finish(c, adga::launch_op_giw(g, c->hk, c->siw, c->dc, d.p, c->stream), "get_giw");
+    if (rc != ADGA_OK) return rc;
+    CK(hipMemcpyAsync(giw, d.p, sizeof(z64) * ne, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ADGA_OK;
+  });
+}
+
+int adga_get_nfock(adga_ctx* c, double* n_fock) {
+  if (!c || !n_fock) return ADGA_E_ARG;
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    const size_t ne = (size_t)g.nkp * g.n2;
+    DevBuf d(ne);
+    const int rc = op_finish(c, adga::launch_op_nfock(g, c->hk, c->siw, c->dc, d.p, c->stream), "get_nfock");
+    if (rc != ADGA_OK) return rc;
+    CK(hipMemcpyAsync(n_fock, d.p, sizeof(z64) * ne, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ADGA_OK;
+  });
+}
+
+int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc, double* gloc) {
+  if (!c || !sigma_sum) return ADGA_E_ARG;
+  const Geom& g = c->g;
+  if (g.nkp_eom != g.nkp) {
+    c->err = "get_sigma_g_loc: sigma_sum must cover the whole k grid (nkp_eom == nkp)";
+    return ADGA_E_ARG;
+  }
+  return guarded(c, [&]() -> int {
+    const size_t ns = (size_t)g.n2 * g.nf * g.nkp, nl = (size_t)g.n2 * g.nf, ng = (size_t)2 * g.iwmax * g.n2;
+    DevBuf ds(ns), dl(nl), dg(ng);
+    upload(ds.p, sigma_sum, ns, c->stream);
+    const int rc = op_finish(
+        c, adga::launch_op_sigma_g_loc(g, c->hk, c->siw, c->dc, ds.p, dl.p, dg.p, c->stream), "get_sigma_g_loc");
+    if (rc != ADGA_OK) return rc;
+    if (sigma_loc) CK(hipMemcpyAsync(sigma_loc, dl.p, sizeof(z64) * nl, hipMemcpyDeviceToHost, c->stream));
+    if (gloc) CK(hipMemcpyAsync(gloc, dg.p, sizeof(z64) * ng, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ADGA_OK;
+  });
+}
+
+int adga_get_ndga(adga_ctx* c, const double* sigma_sum, double* n_dga_k, double* n_dga) {
+  if (!c || !sigma_sum) return ADGA_E_ARG;
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    const size_t ns = (size_t)g.n2 * g.nf * g.nkp_eom, nk = (size_t)g.nkp_eom * g.n2;
+    DevBuf ds(ns), dk(nk), dn((size_t)g.n);
+    upload(ds.p, sigma_sum, ns, c->stream);
+    const int rc = op_finish(
+        c, adga::launch_op_ndga(g, c->hk, c->siw, c->dc, c->keom, ds.p, dk.p, dn.p, c->stream), "get_ndga");
+    if (rc != ADGA_OK) return rc;
+    if (n_dga_k) CK(hipMemcpyAsync(n_dga_k, dk.p, sizeof(z64) * nk, hipMemcpyDeviceToHost, c->stream));
+    if (n_dga) CK(hipMemcpyAsync(n_dga, dn.p, sizeof(z64) * g.n, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
     return ADGA_OK;
   });
 }

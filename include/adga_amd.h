@@ -114,6 +114,43 @@ int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *
 int adga_reset_sigma(adga_ctx *ctx);
 
 /*
+ * One-particle routines around the loop, on the context's one-particle data (hk, siw, dc, mu,
+ * beta, k_data_eom).  G(k, nu)^-1 = (i nu + mu) - H(k) - dc(1,:) - Sigma(k, nu) on the diagonal and
+ * -H(k) - Sigma(k, nu) off it, with nu = pi (2 iw + 1) / beta, iw = 0 .. iwmax-1.  All outputs are
+ * complex(kind=8) arrays in the reference's layouts (first index fastest, frequency offset iwmax or
+ * iwfmax_small); occupations are real-valued complex numbers as in the reference.
+ */
+
+/*
+ * giw(2*iwmax, ndim) = (1/nkp) sum_k G_ii(k, nu) with Sigma = siw, giw(-nu-1) = giw(nu)^*.
+ * Replaces: get_giw (src/one_particle_quant_module.f90:40-79).
+ */
+int adga_get_giw(adga_ctx *ctx, double *giw);
+
+/*
+ * n_fock(nkp, ndim, ndim) = 2/beta sum_{iw=0}^{iwmax-1} Re G(k, nu) + 1/2 on the diagonal, Sigma = siw.
+ * Replaces: get_nfock (src/one_particle_quant_module.f90:337-361).
+ */
+int adga_get_nfock(adga_ctx *ctx, double *n_fock);
+
+/*
+ * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp): the self-energy on the whole k grid after
+ * add_siw_dmft (requires nkp_eom == nkp, as the reference's call does).  Outputs (may be NULL):
+ * sigma_loc(ndim, ndim, 2*iwfmax_small) = its k mean; gloc(2*iwmax, ndim, ndim) = (1/nkp) sum_k G
+ * with Sigma = sigma_sum inside the box (iw < iwfmax_small) and siw outside, gloc(-nu-1) = gloc(nu)^H.
+ * Replaces: get_sigma_g_loc (src/one_particle_quant_module.f90:124-180).
+ */
+int adga_get_sigma_g_loc(adga_ctx *ctx, const double *sigma_sum, double *sigma_loc, double *gloc);
+
+/*
+ * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) as above, on the EOM k list.  Outputs (may be NULL):
+ * n_dga_k(nkp_eom, ndim, ndim) = 2/beta sum_iw Re G(k_data_eom(ik), nu) + 1/2 on the diagonal
+ * (Sigma = sigma_sum inside the box, siw outside); n_dga(ndim) = mean over ik of its diagonal.
+ * Replaces: get_ndga (src/one_particle_quant_module.f90:363-393).
+ */
+int adga_get_ndga(adga_ctx *ctx, const double *sigma_sum, double *n_dga_k, double *n_dga);
+
+/*
  * Statistics of the last adga_run_points call (nout <= 14 values): out[0] = max refinement
  * iterations, out[1] = max final relative residual, out[2..8] = milliseconds spent in the
  * reference's stage buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM, Chi

@@ -20,6 +20,8 @@ shapes and index meaning (first index fastest, 0-based here):
 * EOM (local, static)       eom_module.f90:39-161; main.F90:635-639
 * Sigma assembly            eom_module.f90:237-251; main.F90:840-855
 * chi assembly              main.F90:930-936, 1047-1063, 1124-1140
+* one-particle routines     one_particle_quant_module.f90:40-79 (get_giw), 124-180
+                            (get_sigma_g_loc), 337-361 (get_nfock), 363-393 (get_ndga)
 
 Parity pins (tests/test_oracle_*.py): golden vectors written by the reference's
 own Fortran modules (oracle/_ref, built from the reference sources with
@@ -31,7 +33,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["HotPath", "inverse_small", "kmq_table"]
+__all__ = ["HotPath", "inverse_small", "kmq_table", "get_giw", "get_nfock", "get_sigma_g_loc", "get_ndga"]
 
 
 def inverse_small(a):
@@ -78,6 +80,82 @@ def kmq_table(nk, q_data):
     lz, ly, lx = q % nkz, (q // nkz) % nky, q // (nky * nkz)
     return (((iz[:, None] - lz[None, :]) % nkz) + ((iy[:, None] - ly[None, :]) % nky) * nkz
             + ((ix[:, None] - lx[None, :]) % nkx) * nky * nkz)
+
+
+# ---------------------------------------------------------------- one-particle routines
+def _ginv_1p(inp, ks, iw, sigma=None):
+    """G^-1(k, nu_iw) for 0-based k indices `ks`, shape (len, n, n) with [.., a, b]: -H(k) - Sigma
+    off the diagonal, i nu + mu - H_ii - dc(1, i) - Sigma_ii on it; Sigma = siw(iw, :) (diagonal)
+    when `sigma` is None, else the (len, n, n) matrices given (get_gkiw 81-104, get_gkiw_dga
+    107-121, get_sigma_g_loc 153-160)."""
+    n = inp.wl.n
+    h = np.asarray(inp.hk)[:, :, ks].transpose(2, 0, 1).astype(np.complex128)
+    g = -h if sigma is None else -h - sigma
+    nu = np.pi * (2 * iw + 1) / inp.beta
+    d = np.arange(n)
+    sd = inp.siw[iw + inp.iwmax, :][None, :] if sigma is None else sigma[:, d, d]
+    g[:, d, d] = 1j * nu + inp.mu - h[:, d, d] - inp.dc[0, :][None, :] - sd
+    return g
+
+
+def get_giw(inp):
+    """giw(-iwmax:iwmax-1, n) as a (2*iwmax, n) array: (1/nkp) sum_k diag G(k, nu) for nu >= 0 and
+    giw(-nu-1) = giw(nu)^* (get_giw, one_particle_quant_module.f90:40-79; the reference inverts
+    with inverse_matrix, i.e. LAPACK)."""
+    nkp, off = inp.hk.shape[2], inp.iwmax
+    ks = np.arange(nkp)
+    out = np.zeros((2 * off, inp.wl.n), dtype=np.complex128)
+    for iw in range(off):
+        g = np.linalg.inv(_ginv_1p(inp, ks, iw))
+        out[iw + off] = np.diagonal(g, axis1=1, axis2=2).sum(axis=0)
+        out[-iw - 1 + off] = np.conj(out[iw + off])
+    return out / nkp
+
+
+def get_nfock(inp):
+    """n_fock(nkp, n, n) = 2/beta sum_{iw<iwmax} Re G(k, nu) + 1/2 on the diagonal (get_nfock,
+    one_particle_quant_module.f90:337-361; complex array with zero imaginary part)."""
+    nkp, n = inp.hk.shape[2], inp.wl.n
+    ks = np.arange(nkp)
+    acc = np.zeros((nkp, n, n))
+    for iw in range(inp.iwmax):
+        acc += np.linalg.inv(_ginv_1p(inp, ks, iw)).real
+    out = (2.0 * acc / inp.beta).astype(np.complex128)
+    out[:, np.arange(n), np.arange(n)] += 0.5
+    return out
+
+
+def get_sigma_g_loc(inp, sigma_sum):
+    """(sigma_loc, gloc) of get_sigma_g_loc (one_particle_quant_module.f90:124-180): sigma_sum
+    (n, n, 2*iwfmax_small, nkp) (first index fastest, after add_siw_dmft); sigma_loc its k mean;
+    gloc (2*iwmax, n, n) = (1/nkp) sum_k G with sigma_sum inside the box and siw outside, and
+    gloc(-nu-1) = gloc(nu)^H."""
+    nkp, off, iwf = inp.hk.shape[2], inp.iwmax, inp.wl.iwfmax_small
+    ks = np.arange(nkp)
+    sloc = sigma_sum.sum(axis=3) / nkp
+    gloc = np.zeros((2 * off, inp.wl.n, inp.wl.n), dtype=np.complex128)
+    for iw in range(off):
+        sig = sigma_sum[:, :, iw + iwf, :].transpose(2, 0, 1) if iw < iwf else None
+        gloc[iw + off] = np.linalg.inv(_ginv_1p(inp, ks, iw, sig)).sum(axis=0)
+        gloc[-iw - 1 + off] = np.conj(gloc[iw + off]).T
+    return sloc, gloc / nkp
+
+
+def get_ndga(inp, sigma_sum):
+    """(n_dga_k, n_dga) of get_ndga (one_particle_quant_module.f90:363-393): sigma_sum
+    (n, n, 2*iwfmax_small, nkp_eom) on the EOM k list; n_dga_k(nkp_eom, n, n) = 2/beta sum_iw
+    Re G(k_data_eom(ik), nu) + 1/2 on the diagonal with sigma_sum inside the box and the diagonal
+    siw outside; n_dga the mean of its diagonal."""
+    n, iwf = inp.wl.n, inp.wl.iwfmax_small
+    ks = np.asarray(inp.k_data_eom) - 1
+    acc = np.zeros((len(ks), n, n))
+    for iw in range(inp.iwmax):
+        sig = sigma_sum[:, :, iw + iwf, :].transpose(2, 0, 1) if iw < iwf else None
+        acc += np.linalg.inv(_ginv_1p(inp, ks, iw, sig)).real
+    ndk = (2.0 * acc / inp.beta).astype(np.complex128)
+    ndk[:, np.arange(n), np.arange(n)] += 0.5
+    nd = np.diagonal(ndk, axis1=1, axis2=2).sum(axis=0) / len(ks)
+    return ndk, nd
 
 
 class HotPath:

@@ -74,3 +74,48 @@ def run(inp, points, chi_loc_fn, threads=1, timeout=3600, exe=None, args=()):
     assert o == raw.size, (o, raw.size)
     out["wall_s"] = dt
     return out
+
+
+OP_HARNESS = os.path.join(HERE, "_ref", "adga_op_harness")
+
+
+def op_available() -> bool:
+    return os.path.exists(OP_HARNESS)
+
+
+def run_one_particle(inp, sigma_sum, timeout=3600):
+    """Run the reference's get_giw, get_nfock, get_sigma_g_loc (when nkp_eom == nkp) and get_ndga
+    (oracle/_ref/adga_op_harness) on `inp` and sigma_sum (n, n, 2*iwfmax_small, nkp_eom).
+    Returns a dict of Fortran-shaped complex128 arrays and the wall time."""
+    wl = inp.wl
+    n, nf = wl.n, wl.nf
+    nkp = int(np.prod(wl.nk))
+    nkp_eom = len(inp.k_data_eom)
+    with tempfile.TemporaryDirectory() as td:
+        fi = os.path.join(td, "in.bin")
+        fo = os.path.join(td, "out.bin")
+        with open(fi, "wb") as f:
+            _w(f, [n, *wl.nk, inp.iwmax, wl.iwfmax_small, nkp_eom], np.int32)
+            _w(f, [inp.beta, inp.mu], np.float64)
+            for a in (inp.hk, inp.siw, inp.dc):
+                _w(f, a, np.complex128)
+            _w(f, inp.k_data_eom, np.int32)
+            _w(f, sigma_sum, np.complex128)
+        env = dict(os.environ)
+        env.update(MKL_NUM_THREADS="1", MKL_DYNAMIC="FALSE", OMP_NUM_THREADS="1")
+        import time
+        t0 = time.perf_counter()
+        subprocess.run([OP_HARNESS, fi, fo], check=True, env=env, timeout=timeout)
+        dt = time.perf_counter() - t0
+        raw = np.fromfile(fo, dtype=np.complex128)
+    shapes = [("giw", (2 * inp.iwmax, n)), ("n_fock", (nkp, n, n)), ("sigma_loc", (n, n, nf)),
+              ("gloc", (2 * inp.iwmax, n, n)), ("n_dga_k", (nkp_eom, n, n)), ("n_dga", (n,))]
+    out = {}
+    o = 0
+    for name, shp in shapes:
+        sz = int(np.prod(shp))
+        out[name] = raw[o:o + sz].reshape(shp, order="F")
+        o += sz
+    assert o == raw.size, (o, raw.size)
+    out["wall_s"] = dt
+    return out

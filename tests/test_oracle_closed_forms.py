@@ -143,3 +143,53 @@ def test_bubble_brute_force_tiny():
                                     acc[i, j, l, k] += -inp.beta * g1[i, l] * g2[k, j]
                 ref = (acc / 8).reshape(n * n, n * n)
                 assert rel_err(cq[:, :, d], ref) < 1e-13
+
+
+def _flat_band(eps, mu, beta, iwmax):
+    """One orbital with H(k) = eps on a 2x2x2 grid, Sigma = 0, dc = 0."""
+    wl = synth.workload("sc1_k4", nk=(2, 2, 2), iwmax=iwmax)
+    inp = synth.build_inputs(wl)
+    nkp = 8
+    return dataclasses.replace(inp, hk=np.full((1, 1, nkp), eps, dtype=np.complex128),
+                               siw=np.zeros((2 * iwmax, 1), dtype=np.complex128),
+                               dc=np.zeros((2, 1), dtype=np.complex128), mu=mu, beta=beta)
+
+
+def test_one_particle_flat_band_closed_forms():
+    """A flat band without self-energy: G(nu) = 1/(i nu + mu - eps) at every k, and the
+    occupation 2/beta sum_{n<M} Re G + 1/2 tends to the Fermi function 1/(exp(beta(eps-mu)) + 1);
+    the neglected tail is beta (mu - eps) / (2 pi^2 M) to first order."""
+    eps, mu, beta, M = 0.2, 0.5, 10.0, 20000
+    inp = _flat_band(eps, mu, beta, M)
+    giw = adga_oracle.get_giw(inp)
+    iw = np.arange(-M, M)
+    nu = np.pi * (2 * iw + 1) / beta
+    assert np.abs(giw[:, 0] - 1.0 / (1j * nu + mu - eps)).max() < 1e-15
+    nf = adga_oracle.get_nfock(inp)
+    fermi = 1.0 / (np.exp(beta * (eps - mu)) + 1.0)
+    tail = beta * (mu - eps) / (2 * np.pi ** 2 * M)
+    assert abs(nf[0, 0, 0].real + tail - fermi) < 1e-8
+    assert np.abs(nf.imag).max() == 0.0
+    # the same with the self-energy passed as sigma_sum = 0 inside the box
+    ss = np.zeros((1, 1, inp.wl.nf, 8), dtype=np.complex128)
+    ndk, nd = adga_oracle.get_ndga(inp, ss)
+    assert rel_err(ndk, nf) < 1e-14 and abs(nd[0] - nf[0, 0, 0]) < 1e-14
+    sl, gl = adga_oracle.get_sigma_g_loc(inp, ss)
+    assert np.abs(sl).max() == 0.0 and rel_err(gl[:, 0, 0], giw[:, 0]) < 1e-14
+
+
+def test_one_particle_constant_self_energy_shifts_the_band():
+    """A k- and nu-independent sigma_sum (inside the box) and siw (outside) equal to s act like
+    eps -> eps + s: get_ndga and get_sigma_g_loc equal the self-energy-free routines at the
+    shifted band, and a diagonal dc does the same with the opposite sign of mu."""
+    eps, mu, beta, M, s = -0.3, 0.1, 5.0, 400, 0.25
+    base = _flat_band(eps, mu, beta, M)
+    shifted = _flat_band(eps + s, mu, beta, M)
+    inp = dataclasses.replace(base, siw=np.full((2 * M, 1), s, dtype=np.complex128))
+    ss = np.full((1, 1, inp.wl.nf, 8), s, dtype=np.complex128)
+    ndk, _ = adga_oracle.get_ndga(inp, ss)
+    assert rel_err(ndk, adga_oracle.get_nfock(shifted)) < 1e-13
+    _, gl = adga_oracle.get_sigma_g_loc(inp, ss)
+    assert rel_err(gl[:, 0, 0], adga_oracle.get_giw(shifted)[:, 0]) < 1e-13
+    dcin = dataclasses.replace(base, dc=np.array([[s], [0.0]], dtype=np.complex128))
+    assert rel_err(adga_oracle.get_nfock(dcin), adga_oracle.get_nfock(shifted)) < 1e-13

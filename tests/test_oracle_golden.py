@@ -39,3 +39,25 @@ def test_golden_fixture_files_are_small():
     from conftest import GOLDEN
     tot = sum(os.path.getsize(os.path.join(GOLDEN, f)) for f in os.listdir(GOLDEN) if f.endswith(".npz"))
     assert tot < 8e6
+
+
+ONEP_CASES = ["onep_sc1_k8", "onep_two_k4", "onep_t2g_k4", "onep_d5_k4", "onep_t2g_k4_sub"]
+
+
+@pytest.mark.parametrize("name", ONEP_CASES)
+def test_oracle_one_particle_matches_reference(name):
+    """get_giw, get_nfock, get_sigma_g_loc and get_ndga of the oracle against the reference's own
+    module (tests/golden/make_golden_onep.py), including a reordered subset as the EOM k list."""
+    from golden.make_golden_onep import fixture_inputs, inputs_digest
+    gold = load_golden(name)
+    inp, ss = fixture_inputs(name, gold)
+    assert str(gold["digest"]) == inputs_digest(inp), "synthetic input generator changed"
+    assert rel_err(adga_oracle.get_giw(inp), gold["giw"]) < 1e-12
+    assert rel_err(adga_oracle.get_nfock(inp), gold["n_fock"]) < 1e-12
+    ndk, nd = adga_oracle.get_ndga(inp, ss)
+    assert rel_err(ndk, gold["n_dga_k"]) < 1e-12
+    assert rel_err(nd, gold["n_dga"]) < 1e-12
+    if len(inp.k_data_eom) == inp.hk.shape[2]:
+        sl, gl = adga_oracle.get_sigma_g_loc(inp, ss)
+        assert rel_err(sl, gold["sigma_loc"]) < 1e-12
+        assert rel_err(gl, gold["gloc"]) < 1e-12
