@@ -5,8 +5,10 @@ reference's loop body (src/main.F90:486-799): `set_local_omega` is the local
 update done when omega changes (506-625), `run_points` the per-point work for a
 batch of q points (635-781), `get_sigma` the accumulated self-energies (841-855).
 `run_loop` drives a whole (omega, q) point list the way main.F90 does and
-returns the same quantities the reference accumulates.  `inverse_matrix` mirrors
-lapack_module's inverse_matrix (src/lapack_module.f90:40-70).
+returns the same quantities the reference accumulates.  `get_giw`, `get_nfock`,
+`get_sigma_g_loc` and `get_ndga` are the one-particle routines around the loop
+(src/one_particle_quant_module.f90).  `inverse_matrix` mirrors lapack_module's
+inverse_matrix (src/lapack_module.f90:40-70).
 
 Errors follow the reference's fatal-error behaviour (mpi_stop): a non-zero
 status raises `AdgaError` carrying the status code and the library's message.

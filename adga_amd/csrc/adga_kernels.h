@@ -10,8 +10,9 @@ void launch_gtable(const Geom& g, const z64* hk, const z64* siw, const z64* dc, 
 // one-particle routines (onep.hip); false for an orbital count the kernels do not cover (n > 8)
 bool launch_op_giw(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* giw, hipStream_t s);
 bool launch_op_nfock(const Geom& g, const z64* hk, const z64* siw, const z64* dc, z64* nfock, hipStream_t s);
+size_t op_sigma_loc_scratch(const Geom& g);  // complex elements of the scratch launch_op_sigma_g_loc needs
 bool launch_op_sigma_g_loc(const Geom& g, const z64* hk, const z64* siw, const z64* dc, const z64* ssum, z64* sloc,
-                           z64* gloc, hipStream_t s);
+                           z64* gloc, z64* scratch, hipStream_t s);
 bool launch_op_ndga(const Geom& g, const z64* hk, const z64* siw, const z64* dc, const int* keom, const z64* ssum,
                     z64* ndk, z64* nd, hipStream_t s);
 void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z64* xm, z64* winv, z64* chi0w,

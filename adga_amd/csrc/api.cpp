@@ -565,6 +565,10 @@ int adga_get_giw(adga_ctx* c, double* giw) {
     DevBuf d(ne);
     const int rc = op_finish(c, adga::launch_op_giw(g, c->hk, c->siw, c->dc, d.p, c->stream), "get_giw");
     if (rc != ADGA_OK) return rc;
+    // like get_giw, which overwrites the module's giw: the context's copy (the local bubble's input)
+    // becomes the result, and the local data of the current omega must be formed again
+    CK(hipMemcpyAsync(c->giw, d.p, sizeof(z64) * ne, hipMemcpyDeviceToDevice, c->stream));
+    c->have_omega = false;
     CK(hipMemcpyAsync(giw, d.p, sizeof(z64) * ne, hipMemcpyDeviceToHost, c->stream));
     CK(hipStreamSynchronize(c->stream));
     return ADGA_OK;
@@ -594,10 +598,10 @@ int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc
   }
   return guarded(c, [&]() -> int {
     const size_t ns = (size_t)g.n2 * g.nf * g.nkp, nl = (size_t)g.n2 * g.nf, ng = (size_t)2 * g.iwmax * g.n2;
-    DevBuf ds(ns), dl(nl), dg(ng);
+    DevBuf ds(ns), dl(nl), dg(ng), dw(adga::op_sigma_loc_scratch(g));
     upload(ds.p, sigma_sum, ns, c->stream);
     const int rc = op_finish(
-        c, adga::launch_op_sigma_g_loc(g, c->hk, c->siw, c->dc, ds.p, dl.p, dg.p, c->stream), "get_sigma_g_loc");
+        c, adga::launch_op_sigma_g_loc(g, c->hk, c->siw, c->dc, ds.p, dl.p, dg.p, dw.p, c->stream), "get_sigma_g_loc");
     if (rc != ADGA_OK) return rc;
     if (sigma_loc) CK(hipMemcpyAsync(sigma_loc, dl.p, sizeof(z64) * nl, hipMemcpyDeviceToHost, c->stream));
     if (gloc) CK(hipMemcpyAsync(gloc, dg.p, sizeof(z64) * ng, hipMemcpyDeviceToHost, c->stream));

@@ -155,14 +155,37 @@ __global__ void __launch_bounds__(OP_NT) k_op_ndga_mean(int n, int nk, const z64
   if (t == 0) nd[i] = mk(r.re / nk, 0.0);
 }
 
-// sigma_loc(a, b, iw) = (1/nkp) sum_k sigma_sum(a, b, iw, k) over the whole box (get_sigma_g_loc 145-150);
-// one thread per (a, b, iw).
-__global__ void k_op_sigma_loc(Geom g, const z64* __restrict__ ssum, z64* __restrict__ sloc) {
+// sigma_loc(a, b, iw) = (1/nkp) sum_k sigma_sum(a, b, iw, k) over the whole box (get_sigma_g_loc 145-150),
+// in two passes: partial sums over SL_KCH chunks of k (a wave reads 64 consecutive elements of one
+// k, four waves take every fourth k of the chunk), then the sum of the partials.
+constexpr int SL_KCH = 32;
+
+__global__ void __launch_bounds__(OP_NT) k_op_sigma_loc_part(Geom g, const z64* __restrict__ ssum,
+                                                             z64* __restrict__ part) {
+  __shared__ z64 red[OP_NT / 64][64];
+  const int t = threadIdx.x, el = t & 63, ks = t >> 6;
+  const long ne = (long)g.n2 * g.nf;
+  const long e = (long)blockIdx.x * 64 + el;
+  const int kper = (g.nkp + SL_KCH - 1) / SL_KCH;
+  const int k0 = blockIdx.y * kper, k1 = min(g.nkp, k0 + kper);
+  z64 s = mk(0.0, 0.0);
+  if (e < ne)
+    for (int k = k0 + ks; k < k1; k += OP_NT / 64) s = cadd(s, ssum[(long)k * ne + e]);
+  red[ks][el] = s;
+  __syncthreads();
+  if (ks == 0 && e < ne) {
+    z64 r = red[0][el];
+    for (int w = 1; w < OP_NT / 64; ++w) r = cadd(r, red[w][el]);
+    part[(long)blockIdx.y * ne + e] = r;
+  }
+}
+
+__global__ void k_op_sigma_loc_sum(Geom g, const z64* __restrict__ part, z64* __restrict__ sloc) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ne = (long)g.n2 * g.nf;
   if (e >= ne) return;
   z64 s = mk(0.0, 0.0);
-  for (int k = 0; k < g.nkp; ++k) s = cadd(s, ssum[(long)k * ne + e]);
+  for (int c = 0; c < SL_KCH; ++c) s = cadd(s, part[(long)c * ne + e]);
   sloc[e] = mk(s.re / g.nkp, s.im / g.nkp);
 }
 
@@ -197,10 +220,14 @@ bool launch_op_nfock(const Geom& g, const z64* hk, const z64* siw, const z64* dc
   return true;
 }
 
+size_t op_sigma_loc_scratch(const Geom& g) { return (size_t)SL_KCH * g.n2 * g.nf; }
+
 bool launch_op_sigma_g_loc(const Geom& g, const z64* hk, const z64* siw, const z64* dc, const z64* ssum, z64* sloc,
-                           z64* gloc, hipStream_t s) {
+                           z64* gloc, z64* scratch, hipStream_t s) {
   const long ne = (long)g.n2 * g.nf;
-  hipLaunchKernelGGL(k_op_sigma_loc, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, g, ssum, sloc);
+  hipLaunchKernelGGL(k_op_sigma_loc_part, dim3((unsigned)((ne + 63) / 64), SL_KCH), dim3(OP_NT), 0, s, g, ssum,
+                     scratch);
+  hipLaunchKernelGGL(k_op_sigma_loc_sum, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, g, scratch, sloc);
 #define F(NN) hipLaunchKernelGGL((k_op_ksum<NN, true>), dim3(g.iwmax), dim3(OP_NT), 0, s, g, hk, siw, dc, ssum, gloc)
   ADGA_OP_SWITCH(F)
 #undef F

@@ -8,7 +8,8 @@ module adga_c
   private
   public :: adga_dims, c_adga_ctx_create, c_adga_set_option, c_adga_set_local_omega, c_adga_run_points, &
             c_adga_get_sigma, c_adga_reset_sigma, c_adga_get_stats, c_adga_last_error, c_adga_ctx_destroy, &
-            c_adga_inverse_matrix_z, ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
+            c_adga_inverse_matrix_z, c_adga_get_giw, c_adga_get_nfock, c_adga_get_sigma_g_loc, c_adga_get_ndga, &
+            ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
 
   integer(c_int), parameter :: ADGA_OK = 0, ADGA_E_SINGULAR = 3
   integer(c_int), parameter :: ADGA_PREC_FP64 = 0, ADGA_PREC_REFINED = 1
@@ -90,5 +91,31 @@ module adga_c
        complex(c_double_complex), intent(inout) :: m(*)
        integer(c_int), intent(out) :: ierr
      end function c_adga_inverse_matrix_z
+
+     integer(c_int) function c_adga_get_giw(ctx, giw) bind(c, name='adga_get_giw')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: giw(*)
+     end function c_adga_get_giw
+
+     integer(c_int) function c_adga_get_nfock(ctx, n_fock) bind(c, name='adga_get_nfock')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: n_fock(*)
+     end function c_adga_get_nfock
+
+     integer(c_int) function c_adga_get_sigma_g_loc(ctx, sigma_sum, sigma_loc, gloc) bind(c, name='adga_get_sigma_g_loc')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: sigma_sum(*)
+       complex(c_double_complex), intent(out) :: sigma_loc(*), gloc(*)
+     end function c_adga_get_sigma_g_loc
+
+     integer(c_int) function c_adga_get_ndga(ctx, sigma_sum, n_dga_k, n_dga) bind(c, name='adga_get_ndga')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: sigma_sum(*)
+       complex(c_double_complex), intent(out) :: n_dga_k(*), n_dga(*)
+     end function c_adga_get_ndga
   end interface
 end module adga_c

@@ -9,7 +9,7 @@ module adga_gpu
   implicit none
   private
   public :: adga_gpu_init, adga_gpu_local, adga_gpu_points, adga_gpu_sigma, adga_gpu_free, adga_gpu_option, &
-            ADGA_PREC_FP64, ADGA_PREC_REFINED
+            adga_gpu_giw, adga_gpu_nfock, adga_gpu_sigma_g_loc, adga_gpu_ndga, ADGA_PREC_FP64, ADGA_PREC_REFINED
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: n_ = 0
@@ -76,6 +76,38 @@ contains
     integer, intent(out) :: ierr
     ierr = int(c_adga_get_sigma(ctx, sigma_nl, sigma_dmft, sigma_hf))
   end subroutine adga_gpu_sigma
+
+  ! the one-particle routines of src/one_particle_quant_module.f90 on the context's hk, siw, dc, mu,
+  ! beta and k_data_eom.  giw(-iwmax:iwmax-1,n) (get_giw; also replaces the context's giw)
+  subroutine adga_gpu_giw(giw, ierr)
+    complex(8), intent(out) :: giw(:,:)
+    integer, intent(out) :: ierr
+    ierr = int(c_adga_get_giw(ctx, giw))
+  end subroutine adga_gpu_giw
+
+  ! n_fock(nkp,n,n) (get_nfock)
+  subroutine adga_gpu_nfock(n_fock, ierr)
+    complex(8), intent(out) :: n_fock(:,:,:)
+    integer, intent(out) :: ierr
+    ierr = int(c_adga_get_nfock(ctx, n_fock))
+  end subroutine adga_gpu_nfock
+
+  ! sigma_sum(n,n,-iwfmax_small:iwfmax_small-1,nkp), sigma_loc(n,n,-iwfmax_small:iwfmax_small-1),
+  ! gloc(-iwmax:iwmax-1,n,n) (get_sigma_g_loc)
+  subroutine adga_gpu_sigma_g_loc(sigma_sum, sigma_loc, gloc, ierr)
+    complex(8), intent(in) :: sigma_sum(:,:,:,:)
+    complex(8), intent(out) :: sigma_loc(:,:,:), gloc(:,:,:)
+    integer, intent(out) :: ierr
+    ierr = int(c_adga_get_sigma_g_loc(ctx, sigma_sum, sigma_loc, gloc))
+  end subroutine adga_gpu_sigma_g_loc
+
+  ! sigma_sum(n,n,-iwfmax_small:iwfmax_small-1,nkp_eom), n_dga_k(nkp_eom,n,n), n_dga(n) (get_ndga)
+  subroutine adga_gpu_ndga(sigma_sum, n_dga_k, n_dga, ierr)
+    complex(8), intent(in) :: sigma_sum(:,:,:,:)
+    complex(8), intent(out) :: n_dga_k(:,:,:), n_dga(:)
+    integer, intent(out) :: ierr
+    ierr = int(c_adga_get_ndga(ctx, sigma_sum, n_dga_k, n_dga))
+  end subroutine adga_gpu_ndga
 
   subroutine adga_gpu_free()
     integer(c_int) :: rc

@@ -8,7 +8,9 @@ of the hot path (bubble, BSE build, density + magnetic solves, eta, chi, EOM) ov
 `--batch` q points at one bosonic frequency; the local update (set_local_omega) runs inside the
 timed region whenever the frequency changes.  `value` is refined-mode (complex64 LU on the GPU,
 FP64 residuals and iterative refinement) points/s; the plain FP64 path is timed on the same
-points and reported beside it, with the per-quantity difference between the two.
+points and reported beside it, with the per-quantity difference between the two.  The one-particle
+routines around the loop (get_giw, get_nfock, get_sigma_g_loc, get_ndga) are timed on the same k grid
+with 2000 one-particle frequencies and reported in `one_particle`.
 
 Prints one JSON line.
 """
@@ -124,6 +126,51 @@ def cpu_baseline(synth, inp, chi_cache, npoints):
                       f"{per:.3f} s/point"}
 
 
+def one_particle(api, synth, iwmax=2000, reps=3, cpu_iwmax=200, cpu=True):
+    """The one-particle routines around the loop (get_giw, get_nfock, get_sigma_g_loc, get_ndga) on the
+    t2g_k20 grid with `iwmax` one-particle frequencies (the published run's box): wall time per call
+    (best of `reps`; the two routines that take the self-energy upload it inside the call), and the
+    reference module (oracle/_ref op harness, 1 thread) on the same grid with `cpu_iwmax`
+    frequencies, scaled to `iwmax` (each routine is linear in the frequency count)."""
+    wl = synth.workload("t2g_k20", iwmax=iwmax)
+    inp = synth.build_inputs(wl)
+    n, nkp = wl.n, int(np.prod(wl.nk))
+    rng = np.random.default_rng(5)
+    ss = 0.2 * (rng.normal(size=(n, n, wl.nf, nkp)) + 1j * rng.normal(size=(n, n, wl.nf, nkp)))
+    ss = np.asfortranarray(ss)
+    calls = {"get_giw": lambda c: c.get_giw(), "get_nfock": lambda c: c.get_nfock(),
+             "get_sigma_g_loc": lambda c: c.get_sigma_g_loc(ss), "get_ndga": lambda c: c.get_ndga(ss)}
+    ms = {}
+    with api.Context(inp) as ctx:
+        for name, f in calls.items():
+            f(ctx)
+            best = 1e30
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                f(ctx)
+                best = min(best, time.perf_counter() - t0)
+            ms[name] = round(best * 1e3, 3)
+    inv = 4 * nkp * iwmax
+    out = {"workload": f"t2g_k20 grid (8000 k), iwmax = {iwmax} (k x nu inversions per routine: {nkp * iwmax})",
+           "ms": ms, "total_ms": round(sum(ms.values()), 3),
+           "inversions_per_s": round(inv / (sum(ms.values()) / 1e3), 1),
+           "note": "wall time per call through the C ABI; get_sigma_g_loc and get_ndga include the upload of "
+                   "sigma_sum (n^2 x 2 iwfmax_small x nkp complex, 138 MB) from host memory"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_runner
+    if cpu and ref_runner.op_available():
+        cwl = synth.workload("t2g_k20", iwmax=cpu_iwmax)
+        cinp = synth.build_inputs(cwl)
+        r = ref_runner.run_one_particle(cinp, ss)
+        scaled = r["wall_s"] * iwmax / cpu_iwmax
+        out["cpu_baseline"] = {"ms": round(scaled * 1e3, 1), "cores": 1, "kind": "reference",
+                               "sample": f"the four routines of src/one_particle_quant_module.f90 (oracle/_ref, "
+                                         f"MKL, 1 thread) on the same grid with iwmax = {cpu_iwmax}: "
+                                         f"{r['wall_s']:.2f} s, scaled by {iwmax}/{cpu_iwmax}"}
+        out["gpu_over_cpu"] = round(scaled * 1e3 / out["total_ms"], 1)
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes` (reads and writes `nbytes` each), outside any timed region."""
     import torch
@@ -147,6 +194,10 @@ def main():
     ap.add_argument("--cpu-points", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-one-particle", action="store_true", help="skip the one-particle routines' timing")
+    ap.add_argument("--only-one-particle", action="store_true",
+                    help="time only the one-particle routines (get_giw, get_nfock, get_sigma_g_loc, get_ndga) "
+                         "and print their JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device and exit: a kernel of known HBM traffic whose "
                          "hardware byte counters scale those of the hot-path kernels")
@@ -160,6 +211,9 @@ def main():
         copy_calibration()
         return
     from adga_amd import api, synth
+    if args.only_one_particle:
+        print(json.dumps({"one_particle": one_particle(api, synth, cpu=not args.no_cpu_baseline)}), flush=True)
+        return
     wl = synth.workload(args.workload)
     inp = synth.build_inputs(wl)
     omegas = [0, 1, -1, 2, -2, 3, -3]
@@ -228,6 +282,8 @@ def main():
         line["rel_err_refined_vs_fp64"] = errs
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(synth, inp, chi_cache, args.cpu_points)
+    if not args.no_one_particle:
+        line["one_particle"] = one_particle(api, synth, cpu=not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
 
 

@@ -122,7 +122,9 @@ int adga_reset_sigma(adga_ctx *ctx);
  */
 
 /*
- * giw(2*iwmax, ndim) = (1/nkp) sum_k G_ii(k, nu) with Sigma = siw, giw(-nu-1) = giw(nu)^*.
+ * giw(2*iwmax, ndim) = (1/nkp) sum_k G_ii(k, nu) with Sigma = siw, giw(-nu-1) = giw(nu)^*.  As get_giw
+ * overwrites the module's giw, the context's giw (the local bubble's input) becomes the result:
+ * adga_set_local_omega must be called again before the next adga_run_points (ADGA_E_STATE otherwise).
  * Replaces: get_giw (src/one_particle_quant_module.f90:40-79).
  */
 int adga_get_giw(adga_ctx *ctx, double *giw);

@@ -61,3 +61,26 @@ def test_gpu_one_particle_full_grid_against_oracle():
         sl, gl = ctx.get_sigma_g_loc(ss)
         osl, ogl = adga_oracle.get_sigma_g_loc(inp, ss)
         assert rel_err(sl, osl) < TOL and rel_err(gl, ogl) < TOL
+
+
+def test_gpu_get_giw_replaces_the_context_giw():
+    """A context created with giw = 0 gets the lattice giw from adga_get_giw, as get_giw
+    overwrites the module's giw (main.F90:207); the loop then matches the reference's golden
+    vectors (whose giw is the same lattice sum), and after another adga_get_giw the next
+    adga_run_points needs adga_set_local_omega first (ADGA_E_STATE)."""
+    from adga_amd import api
+    from golden.make_golden import FIXTURES
+    gold = load_golden("t2g_k4")
+    wl = synth.workload(FIXTURES["t2g_k4"][0])
+    inp = synth.build_inputs(wl)
+    pts = [tuple(p) for p in gold["points"].tolist()]
+    print("one-particle get_giw into the context (t2g_k4 loop)", flush=True)
+    with api.Context(dataclasses.replace(inp, giw=np.zeros_like(inp.giw))) as ctx:
+        assert rel_err(ctx.get_giw(), inp.giw) < 1e-13
+        out = api.run_loop(ctx, pts, lambda w, c: synth.chi_loc(inp, w, c), "fp64")
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+            assert rel_err(out[k], gold[k]) < 1e-10, k
+        ctx.get_giw()
+        with pytest.raises(api.AdgaError) as e:
+            ctx.run_points(np.array([1], dtype=np.int32))
+        assert e.value.code == 6  # ADGA_E_STATE
