@@ -23,7 +23,7 @@ struct adga_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
   hipEvent_t ev_lag = nullptr, ev_join = nullptr;
-  int lu_split = 1;                        // option "lu_streams": 2 splits each LU batch over two streams
+  int lu_split = 2;                        // option "lu_streams": 2 splits each LU batch over two streams
   std::string err;
   int refine_maxit = 12;
   double refine_tol = 1e-12;

@@ -895,8 +895,11 @@ bool lu_op(int nmat, int N, int W, int ldm, const int* op, cplx<T>* M, int* piv,
 }  // namespace
 
 // With a second stream the batch is split in two halves, the second one operation behind the
-// first on its own stream: while one half runs a latency-bound panel the other's Schur update can
-// use the matrix cores.  `lag` and `join` are events the caller owns.
+// first on its own stream: while one half runs a latency-bound panel or a small update, the other
+// half's operation can use the rest of the machine.  The large Schur updates (those with 128x128
+// tiles, the dominant kernel) run on the whole batch on the first stream, between a join and a
+// fork, so they own the machine; after each, the second stream starts one operation behind again.
+// `lag` and `join` are events the caller owns.
 template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
                int* info, hipStream_t s, LuTimer* timer, hipStream_t s2, hipEvent_t lag, hipEvent_t join) {
@@ -908,10 +911,11 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
   cplx<T>* ib = inv + (long)na * istride;
   int* pb = piv + (long)na * N;
   int* fb = info + na;
+  auto sync = [](hipEvent_t e, hipStream_t from, hipStream_t to) {
+    return hipEventRecord(e, from) == hipSuccess && hipStreamWaitEvent(to, e, 0) == hipSuccess;
+  };
   hipLaunchKernelGGL(k_lu_init, dim3((nmat + 255) / 256), dim3(256), 0, s, nmat, info);
-  if (split) {
-    if (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess) return false;
-  }
+  if (split && !sync(lag, s, s2)) return false;
   bool uinv = false;
   auto launch_uinv = [&]() {
     const int np = (N + P - 1) / P;
@@ -919,20 +923,30 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
     if (split) hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, nb), dim3(64), 0, s2, N, ldm, mstride, istride, Mb, ib);
     uinv = true;
   };
+  bool relag = true;  // the second stream waits for the first one's next operation
   for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
-    if (ops[o] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
-    if (!lu_op<T>(na, N, W, ldm, &ops[o], M, piv, inv, info, s, timer)) return false;
+    const int* op = &ops[o];
+    if (op[0] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
+    const bool big = op[0] == OP_GEMM && op[2] - op[1] > 64 && op[4] - op[3] > 64;
+    if (split && big) {
+      if (!sync(join, s2, s)) return false;
+      if (!lu_op<T>(nmat, N, W, ldm, op, M, piv, inv, info, s, timer)) return false;
+      if (!sync(lag, s, s2)) return false;
+      relag = true;
+      continue;
+    }
+    if (!lu_op<T>(na, N, W, ldm, op, M, piv, inv, info, s, timer)) return false;
     if (split) {
-      if (o == 0 && (hipEventRecord(lag, s) != hipSuccess || hipStreamWaitEvent(s2, lag, 0) != hipSuccess))
-        return false;
-      if (!lu_op<T>(nb, N, W, ldm, &ops[o], Mb, pb, ib, fb, s2, nullptr)) return false;
+      if (relag && !sync(lag, s, s2)) return false;
+      relag = false;
+      if (!lu_op<T>(nb, N, W, ldm, op, Mb, pb, ib, fb, s2, nullptr)) return false;
     }
   }
   if (!uinv) launch_uinv();
   hipLaunchKernelGGL(k_lu_perm, dim3(na), dim3(NT), 0, s, N, piv);
   if (split) {
     hipLaunchKernelGGL(k_lu_perm, dim3(nb), dim3(NT), 0, s2, N, pb);
-    if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return false;
+    if (!sync(join, s2, s)) return false;
   }
   return true;
 }

@@ -48,11 +48,11 @@ def read_traffic(name):
     return None
 
 
-def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache):
+def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache, lu_streams=None):
     wl = inp.wl
     nqp = wl.nqp
     blocks_per_omega = math.ceil(nqp / batch)
-    ctx = api.Context(inp, batch=batch)
+    ctx = api.Context(inp, batch=batch, lu_streams=lu_streams)
     cur = [None]
 
     def step(s):
@@ -194,6 +194,7 @@ def main():
     ap.add_argument("--cpu-points", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--lu-streams", type=int, default=None, help="the library's lu_streams option (1 or 2)")
     ap.add_argument("--no-one-particle", action="store_true", help="skip the one-particle routines' timing")
     ap.add_argument("--only-one-particle", action="store_true",
                     help="time only the one-particle routines (get_giw, get_nfock, get_sigma_g_loc, get_ndga) "
@@ -229,9 +230,9 @@ def main():
     for s in range(nsteps):
         chi_cache(omegas[(s // blocks) % len(omegas)])
 
-    ref = run_mode(api, synth, inp, "refined", args.batch, args.steps, args.warmup, omegas, chi_cache)
+    ref = run_mode(api, synth, inp, "refined", args.batch, args.steps, args.warmup, omegas, chi_cache, args.lu_streams)
     fp = None if args.no_fp64 else run_mode(api, synth, inp, "fp64", args.batch, args.steps, args.warmup,
-                                             omegas, chi_cache)
+                                             omegas, chi_cache, args.lu_streams)
     value = ref["points"] / ref["seconds"]
     N = wl.maxdim
     nmat = 2 * args.batch

@@ -70,7 +70,9 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
  * "bubble" (0: direct k sum per point; 1: all q of the current omega at once by
  * k-grid DFT, cached per omega; 2: DFT when a batch has >= 64 points or the cache is valid, the
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
- * G by k-grid DFT when omega changes or the self-energy is read, the default).
+ * G by k-grid DFT when omega changes or the self-energy is read, the default), "lu_streams" (1: each
+ * batch's LU on one stream; 2: the batch split in two halves on two streams, the second one
+ * operation behind, the large Schur updates on the whole batch; the default; results are identical).
  * Returns ADGA_E_ARG for an unknown key or value.
  */
 int adga_set_option(adga_ctx *ctx, const char *key, double value);

@@ -157,6 +157,23 @@ def test_gpu_refined_equals_fp64_on_a_batch(t2g_full):
         assert rel_err(res["refined"][k], res["fp64"][k]) < 1e-10, k
 
 
+def test_gpu_two_stream_lu_is_identical(t2g_full):
+    """lu_streams = 2 (the batch's LU split over two streams, the large Schur updates on the whole
+    batch) runs the same kernels on the same data: results equal the one-stream run exactly, in
+    both modes, on a 96-point batch (above the split threshold of 16 matrices)."""
+    api = _api()
+    inp = t2g_full
+    pts = [(1, q) for q in range(5, 8000, 83)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    for precision in ("fp64", "refined"):
+        res = {}
+        for ls in (1, 2):
+            with api.Context(inp, lu_streams=ls) as ctx:
+                res[ls] = api.run_loop(ctx, pts, cl, precision)
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
+            assert np.array_equal(res[1][k], res[2][k]), (precision, k)
+
+
 def test_gpu_five_orbital_large_box_against_oracle():
     """5-orbital d-shell model (random symmetric hopping), 16^3 k, 2x80 fermionic frequencies: BSE
     N = 4000, the largest size the batched LU takes (16 rows per thread in the panels; the solves
