@@ -157,6 +157,40 @@ def test_gpu_refined_equals_fp64_on_a_batch(t2g_full):
         assert rel_err(res["refined"][k], res["fp64"][k]) < 1e-10, k
 
 
+# (workload, vertex noise, overrides): noise 30 (30000x the default) gives condition numbers
+# 1e3-1e4 at N = 108 and 200 with most rows swapped; at N = 1620 the same noise reaches 1e7, where
+# any two FP64 formulations differ by up to cond * eps ~ 1e-9, so that case uses noise 0.3
+# (condition ~ 4e2, still ~400 row swaps).
+PIVOT_CASES = [("t2g_k4", 30.0, {}), ("d5_k4", 30.0, {}), ("t2g_k4", 0.3, {"iwfmax_small": 90})]
+
+
+@pytest.mark.parametrize("name,noise,over", PIVOT_CASES, ids=["t2g_N108", "d5_N200", "t2g_N1620"])
+def test_gpu_strong_vertex_pivoting_against_oracle(name, noise, over):
+    """A strong vertex makes A = 1 - S C far from the identity (partial pivoting swaps most rows):
+    both LU precisions' pivoting paths (complex64 leaves of 8 columns, complex128 of 4; 256- and,
+    at N = 1620, 512-thread panels) and several refinement iterations, against the oracle.  The
+    refined mode runs with refine_tol = 1e-14 so that its forward error stays below 1e-10 at
+    this conditioning."""
+    import adga_oracle
+    api = _api()
+    wl = synth.workload(name, noise=noise, **over)
+    inp = synth.build_inputs(wl)
+    pts = [(1, 0), (1, 3), (1, 17)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    ref = adga_oracle.HotPath(inp).run(pts, cl)
+    for precision in ("fp64", "refined"):
+        print(f"strong vertex {name} N={wl.n ** 2 * wl.nf} {precision}", flush=True)
+        with api.Context(inp, refine_tol=1e-14) as ctx:
+            out = api.run_loop(ctx, pts, cl, precision)
+            st = ctx.stats()
+        if precision == "refined":
+            assert st["refine_iters"] >= 2
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+            e = rel_err(out[k], ref[k])
+            print(f"  {k} {e:.2e}", flush=True)
+            assert e < 1e-10, (precision, k, e)
+
+
 def test_gpu_two_stream_lu_is_identical(t2g_full):
     """lu_streams = 2 (the batch's LU split over two streams, the large Schur updates on the whole
     batch) runs the same kernels on the same data: results equal the one-stream run exactly, in
