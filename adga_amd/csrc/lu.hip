@@ -554,16 +554,17 @@ __global__ void __launch_bounds__(NT) k_lu_perm(int N, int* __restrict__ pivall)
 // resident solution (16 lanes per row, shuffle reduction), the diagonal block through its stored
 // inverse.  Wave w owns rows 8w .. 8w+7 of a block row, lane l rows 8w + l/16 and 8w + 4 + l/16 at
 // columns l%16 + 16j.
-constexpr int SOLVE_LDS_BYTES = 80896;  // two workgroups per CU
+constexpr int SOLVE_LDS_BYTES = 80896;        // two workgroups per CU
+constexpr int SOLVE_LDS_BYTES_WIDE = 163840;  // one workgroup per CU (the whole LDS)
 
-template <typename T, int CG>
+template <typename T, int CG, int LDSB>
 __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ldm, const cplx<T>* __restrict__ Mall,
                                                 const int* __restrict__ permall, const cplx<T>* __restrict__ invall,
                                                 const z64* __restrict__ R, z64* __restrict__ Y, int accumulate) {
   using C = cplx<T>;
   // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
   // columns without predicates
-  __shared__ C x[SOLVE_LDS_BYTES / sizeof(C)];
+  __shared__ C x[LDSB / sizeof(C)];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
   const int m = blockIdx.x, s0 = blockIdx.y * ns, nsg = min(ns, nrhs - s0);
   const C* M = Mall + (long)m * N * ldm;
@@ -980,32 +981,50 @@ LuTimer::~LuTimer() {
 }
 
 // columns per group: one group when a solution of nrhs columns (padded to the kernel's column count)
-// fits the LDS, else groups as wide as fit; false when not even one column fits
+// fits the LDS at two workgroups per CU; else one group at one workgroup per CU when it fits the
+// whole LDS (the factors are then read once instead of once per group); else groups as wide as fit
+// at two workgroups per CU.  False when not even one column fits.
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s) {
-  const int cap = SOLVE_LDS_BYTES / (int)sizeof(cplx<T>) / (N + P);  // columns that fit, with P zero rows
   static const int cgs[] = {1, 2, 4, 9, 16};
-  int cg = 0;
-  for (int c : cgs)
-    if (c <= cap && c >= nrhs) { cg = c; break; }
-  if (!cg)
+  auto pick = [&](int ldsb, bool whole) {
+    const int cap = ldsb / (int)sizeof(cplx<T>) / (N + P);  // columns that fit, with P zero rows
     for (int c : cgs)
-      if (c <= cap) cg = c;
+      if (c <= cap && c >= nrhs) return c;
+    int cg = 0;
+    if (!whole)
+      for (int c : cgs)
+        if (c <= cap) cg = c;
+    return cg;
+  };
+  int cg = pick(SOLVE_LDS_BYTES, true);
+  bool wide = false;
+  if (!cg) {
+    cg = pick(SOLVE_LDS_BYTES_WIDE, true);
+    wide = cg == 4 || cg == 9 || cg == 16;
+    if (!wide) cg = pick(SOLVE_LDS_BYTES, false);
+  }
   if (!cg) return false;
   const int ns = std::min(nrhs, cg), ng = (nrhs + ns - 1) / ns;
   dim3 grid(nmat, ng);
-#define ADGA_SOLVE_CASE(CGV)                                                                             \
-  if (cg == CGV) {                                                                                       \
-    hipLaunchKernelGGL((k_lu_solve<T, CGV>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
-                       accumulate ? 1 : 0);                                                              \
-    return true;                                                                                         \
+#define ADGA_SOLVE_CASE(CGV, LDSB)                                                                              \
+  if (cg == CGV) {                                                                                              \
+    hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
+                       accumulate ? 1 : 0);                                                                     \
+    return true;                                                                                                \
   }
-  ADGA_SOLVE_CASE(1)
-  ADGA_SOLVE_CASE(2)
-  ADGA_SOLVE_CASE(4)
-  ADGA_SOLVE_CASE(9)
-  ADGA_SOLVE_CASE(16)
+  if (wide) {
+    ADGA_SOLVE_CASE(4, SOLVE_LDS_BYTES_WIDE)
+    ADGA_SOLVE_CASE(9, SOLVE_LDS_BYTES_WIDE)
+    ADGA_SOLVE_CASE(16, SOLVE_LDS_BYTES_WIDE)
+    return false;
+  }
+  ADGA_SOLVE_CASE(1, SOLVE_LDS_BYTES)
+  ADGA_SOLVE_CASE(2, SOLVE_LDS_BYTES)
+  ADGA_SOLVE_CASE(4, SOLVE_LDS_BYTES)
+  ADGA_SOLVE_CASE(9, SOLVE_LDS_BYTES)
+  ADGA_SOLVE_CASE(16, SOLVE_LDS_BYTES)
 #undef ADGA_SOLVE_CASE
   return false;
 }
