@@ -70,6 +70,7 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
     ctx.reset_sigma()
     cur[0] = None  # run_points returns with its stream drained, so the device is idle here
     lu_ms, lu_n, iters, resid = 0.0, 0, 0.0, 0.0
+    lu_mats = 0
     g_ms, g_flops, g_n = 0.0, 0.0, 0
     outs = []
     npts = 0
@@ -79,6 +80,7 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
         st = ctx.stats()
         lu_ms += st["t_lu_kernel"]
         lu_n += int(st["n_lu_launches"])
+        lu_mats += 2 * r["chi_d"].shape[2]  # density + magnetic matrix per point
         g_ms += st["t_gemm_dominant"]
         g_flops += st["flops_gemm_dominant"]
         g_n += int(st["n_gemm_dominant"])
@@ -89,7 +91,7 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
     sigma = ctx.get_sigma()[0]  # completes the EOM of the last frequency (inside the timed region)
     t1 = time.perf_counter()
     ctx.close()
-    return {"seconds": t1 - t0, "points": npts, "lu_ms": lu_ms, "lu_launches": lu_n, "iters": iters,
+    return {"seconds": t1 - t0, "points": npts, "lu_ms": lu_ms, "lu_launches": lu_n, "lu_mats": lu_mats, "iters": iters,
             "gemm_ms": g_ms, "gemm_flops": g_flops, "gemm_launches": g_n,
             "resid": resid, "chi_d": np.concatenate([o["chi_d"] for o in outs], axis=2),
             "chi_m": np.concatenate([o["chi_m"] for o in outs], axis=2), "sigma_nl": sigma}
@@ -189,7 +191,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=2048)
     ap.add_argument("--workload", default="t2g_k20")
     ap.add_argument("--cpu-points", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -236,8 +238,10 @@ def main():
     value = ref["points"] / ref["seconds"]
     N = wl.maxdim
     nmat = 2 * args.batch
-    lu_fl = nmat * lu_flops(N)
+    lu_fl = nmat * lu_flops(N)  # one full batch
     avg_lu_s = ref["lu_ms"] / max(ref["lu_launches"], 1) / 1e3
+    # achieved rates over the matrices actually factored (an omega's last batch can be partial)
+    lu_rate = ref["lu_mats"] * lu_flops(N) / (ref["lu_ms"] / 1e3) / 1e12 if ref["lu_ms"] > 0 else 0.0
     g_avg_ms = ref["gemm_ms"] / max(ref["gemm_launches"], 1)
     g_avg_flops = ref["gemm_flops"] / max(ref["gemm_launches"], 1)
     achieved = g_avg_flops / (g_avg_ms / 1e3) / 1e12 if g_avg_ms > 0 else 0.0
@@ -263,22 +267,23 @@ def main():
                      "kernel": "k_gemm<float,128,128,2,2> (batched complex Schur update of the LU, 128x128 tiles)",
                      "flops_per_launch": g_avg_flops, "avg_launch_ms": round(g_avg_ms, 4),
                      "launches_timed": ref["gemm_launches"]},
-        "lu_factorisation": {"achieved_tflops": round(lu_fl / avg_lu_s / 1e12, 3), "avg_ms": round(avg_lu_s * 1e3, 3),
-                             "matrices": nmat, "flops": lu_fl},
+        "lu_factorisation": {"achieved_tflops": round(lu_rate, 3), "avg_ms": round(avg_lu_s * 1e3, 3),
+                             "matrices_per_batch": nmat, "flops_per_batch": lu_fl,
+                             "matrices_timed": ref["lu_mats"], "factorisations_timed": ref["lu_launches"]},
         "refine_iters_max": ref["iters"],
         "refine_resid_max": ref["resid"],
     }
     if fp is not None:
         fval = fp["points"] / fp["seconds"]
-        favg = fp["lu_ms"] / max(fp["lu_launches"], 1) / 1e3
+        frate = fp["lu_mats"] * lu_flops(N) / (fp["lu_ms"] / 1e3) / 1e12 if fp["lu_ms"] > 0 else 0.0
         errs = {}
         for k in ("chi_d", "chi_m", "sigma_nl"):
             den = np.abs(fp[k]).max()
             errs[k] = float(np.abs(ref[k] - fp[k]).max() / den) if den > 0 else 0.0
         line["plain_fp64"] = {"value": round(fval, 3), "unit": "points/s",
                               "ms_per_step": round(1e3 * fp["seconds"] / args.steps, 3),
-                              "lu_achieved_tflops": round(lu_fl / favg / 1e12, 3),
-                              "lu_frac_of_fp64_peak": round(lu_fl / favg / 1e12 / PEAK["fp64"], 4)}
+                              "lu_achieved_tflops": round(frate, 3),
+                              "lu_frac_of_fp64_peak": round(frate / PEAK["fp64"], 4)}
         line["refined_over_fp64"] = round(value / fval, 3)
         line["rel_err_refined_vs_fp64"] = errs
     if not args.no_cpu_baseline:
