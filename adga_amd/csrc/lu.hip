@@ -468,36 +468,37 @@ __device__ __forceinline__ void wg_panel(cplx<T>* __restrict__ M, int ldm, int N
 }
 
 // U11^-1 of every panel of every matrix (grid: panels x matrices), after the factorisation: column c
-// of the inverse by back substitution, one lane per column.
+// of the inverse by back substitution in lane c, fully unrolled so the column stays in registers
+// and U11 is read from LDS by all lanes at once; y_a = 0 for a > c follows from the recurrence
+// itself.  Padding rows of a narrow last panel get a unit diagonal.
 template <typename T>
 __global__ void __launch_bounds__(64) k_lu_uinv(int N, int ldm, long mstride, long istride,
                                                 const cplx<T>* __restrict__ Mall, cplx<T>* __restrict__ invall) {
   using C = cplx<T>;
-  __shared__ C Ub[P][P + 1], Xu[P][P + 1];
+  __shared__ C Ub[P][P + 1];
+  __shared__ C rd[P];
   const int t = threadIdx.x, c0 = blockIdx.x * P, w = min(P, N - c0);
   const C* M = Mall + (long)blockIdx.y * mstride;
   for (int e = t; e < P * P; e += 64) {
     const int a = e / P, b = e % P;
-    Ub[a][b] = (a < w && b < w) ? M[(long)(c0 + a) * ldm + c0 + b] : mk<T>(0, 0);
+    Ub[a][b] = (a < w && b < w) ? M[(long)(c0 + a) * ldm + c0 + b] : mk<T>(a == b ? 1 : 0, 0);
   }
   __syncthreads();
-  if (t < w) {
+  if (t < P) rd[t] = crecip(Ub[t][t]);
+  __syncthreads();
+  if (t < P) {
     const int c = t;
-    for (int a = w - 1; a >= 0; --a) {
-      C y = mk<T>(0, 0);
-      if (a <= c) {
-        y = mk<T>(a == c ? 1 : 0, 0);
-        for (int b = a + 1; b <= c; ++b) y = cfms(Ub[a][b], Xu[b][c], y);
-        y = cdiv(y, Ub[a][a]);
-      }
-      Xu[a][c] = y;
+    C y[P];
+#pragma unroll
+    for (int a = P - 1; a >= 0; --a) {
+      C acc = mk<T>(a == c ? 1 : 0, 0);
+#pragma unroll
+      for (int b = a + 1; b < P; ++b) acc = cfms(Ub[a][b], y[b], acc);
+      y[a] = cmul(acc, rd[a]);
     }
-  }
-  __syncthreads();
-  C* dst = invall + (long)blockIdx.y * istride + (long)blockIdx.x * 2 * P * P + P * P;
-  for (int e = t; e < P * P; e += 64) {
-    const int a = e / P, b = e % P;
-    dst[e] = (a < w && b < w) ? Xu[a][b] : mk<T>(0, 0);
+    C* dst = invall + (long)blockIdx.y * istride + (long)blockIdx.x * 2 * P * P + P * P;
+#pragma unroll
+    for (int a = 0; a < P; ++a) dst[a * P + c] = (a < w && c < w) ? y[a] : mk<T>(0, 0);
   }
 }
 
