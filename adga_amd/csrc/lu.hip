@@ -60,6 +60,23 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
   }
 }
 
+// The large Schur updates (mode 0, 128 x 128 tiles) with the 3M product: eight waves, one
+// workgroup per CU (two K-chunk buffers in LDS), the same XCD-aware tile decode as k_gemm.
+constexpr int G3_KC = 32;
+template <typename T, int TMT, int TNT, int WR, int WC>
+__global__ void __launch_bounds__(WR * WC * 64) k_gemm3m(int nmat, int ntc, int ntile, long mstride,
+                                                         cplx<T>* __restrict__ Mall, int ldm, int r0, int m, int c0,
+                                                         int n, int k0, int K) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<T, TMT, TNT, WR, WC, G3_KC>::LDS];
+  const int L = blockIdx.x, x = L & 7, sl = L >> 3;
+  const int mat = (sl / ntile) * 8 + x, tile = sl % ntile;
+  if (mat >= nmat) return;
+  const int rt = (tile / ntc) * TMT, ct = (tile % ntc) * TNT;
+  cplx<T>* M = Mall + (long)mat * mstride;
+  gemm3m_tile<T, TMT, TNT, WR, WC, G3_KC>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm,
+                                          M + (long)r0 * ldm + c0, ldm, m, n, K, false, rt, ct, lds);
+}
+
 // component-wise select of two complex values (a select of whole structs can turn into a select of
 // their addresses, which forces register arrays into scratch memory)
 template <typename T>
@@ -820,6 +837,16 @@ void gemm_region(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int
   if (timed) {
     timer->mark(s);
     timer->flops += 8.0 * m * (double)n * K * nmat;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (tm == 128 && tn == 128 && mode == 0) {
+      const int ntr = (m + 127) / 128, ntc = (n + 127) / 128, ntile = ntr * ntc;
+      const long nwg = (long)ntile * ((nmat + 7) / 8) * 8;
+      hipLaunchKernelGGL((k_gemm3m<T, 128, 128, 2, 4>), dim3((unsigned)nwg), dim3(512), 0, s, nmat, ntc, ntile,
+                         mstride, M, ldm, r0, m, c0, n, k0, K);
+      if (timed) timer->mark(s);
+      return;
+    }
   }
 #define ADGA_G(TMT, TNT, WR, WC) \
   gemm_launch<T, TMT, TNT, WR, WC>(nmat, mstride, M, ldm, r0, m, c0, n, k0, K, mode, inv, istride, s)

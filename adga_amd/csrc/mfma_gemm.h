@@ -290,4 +290,159 @@ __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda
       }
 }
 
+
+// ---------------------------------------------------------------- 3M tile (three real products)
+// The row-major product of gemm_tile with the complex product formed from three real products on
+// the MFMA instead of four: X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi), Re(AB) = X - Y,
+// Im(AB) = Z - X - Y (the 3M method: normwise stable, the imaginary part's error bounded by
+// u (|Ar| + |Ai|)(|Br| + |Bi|) instead of u |A||B|; N. J. Higham, SIAM J. Matrix Anal. Appl. 13
+// (1992) 681-687).  Each MFMA takes four consecutive k, lane l supplying k = l/16.  WR x WC waves
+// (the workgroup is WR * WC * 64 threads); two LDS buffers, chunk kb + KC staged into one while
+// chunk kb is multiplied out of the other, so each chunk costs one barrier.  C is read only in
+// the epilogue, after the last barrier.
+template <typename T, int TMT, int TNT, int WR, int WC, int KC>
+struct G3Cfg {
+  static constexpr int NTH = WR * WC * 64;
+  static constexpr int CS = (int)sizeof(cplx<T>);
+  static constexpr int VEC = 16 / CS;        // complex per 16-byte load
+  static constexpr int LDA = KC + 16 / CS;   // As row stride (complex): the 16 lanes of a K slot hit distinct banks
+  static constexpr int LDB = TNT + 128 / CS;  // Bs row stride (complex): K slots 0/1 in opposite bank halves
+  static constexpr int BUF = TMT * LDA + KC * LDB;  // complex per buffer
+  static constexpr int LDS = 2 * BUF * CS;
+};
+
+template <typename T, int TMT, int TNT, int WR, int WC, int KC>
+__device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int lda, const cplx<T>* B, int ldb,
+                                            cplx<T>* Cm, int ldc, int m, int n, int K, bool assign, int rt, int ct,
+                                            unsigned char* lds) {
+  using V = typename Mfma<T>::V;
+  using C = cplx<T>;
+  using Cfg = G3Cfg<T, TMT, TNT, WR, WC, KC>;
+  constexpr int NTH = Cfg::NTH, VEC = Cfg::VEC, LDA = Cfg::LDA, LDB = Cfg::LDB, BUF = Cfg::BUF;
+  constexpr int WTM = TMT / WR, WTN = TNT / WC, RB = WTM / 16, CB = WTN / 16;
+  static_assert(RB >= 1 && CB >= 1 && WTM % 16 == 0 && WTN % 16 == 0, "wave tile of 16x16 blocks");
+  constexpr int AV = KC / VEC, BV = TNT / VEC;  // 16-byte vectors per A row / B row of a chunk
+  constexpr int PA = TMT * AV / NTH, PB = KC * BV / NTH;
+  static_assert(KC % 4 == 0 && KC % VEC == 0 && PA >= 1 && PB >= 1 && (TMT * AV) % NTH == 0 && (KC * BV) % NTH == 0,
+                "K chunk");
+  struct alignas(16) CV { C v[VEC]; };
+  C* const L = reinterpret_cast<C*>(lds);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wr = (wave / WC) * WTM, wc = (wave % WC) * WTN;
+  const int li = lane & 15, lg = lane >> 4;
+  const bool aligned = ((reinterpret_cast<size_t>(A) | reinterpret_cast<size_t>(B)) % 16 == 0) &&
+                       (lda % VEC == 0) && (ldb % VEC == 0);
+  const bool inside = aligned && rt + TMT <= m && ct + TNT <= n;
+  V x[RB][CB], y[RB][CB], z[RB][CB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[rb][cb][v] = y[rb][cb][v] = z[rb][cb][v] = (T)0;
+  CV pa[PA], pb[PB];
+  auto fetch = [&](int kb) {
+    if (inside && kb + KC <= K) {
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        const int e = t + i * NTH, row = e / AV, k = (e % AV) * VEC;
+        pa[i] = *reinterpret_cast<const CV*>(A + (rt + row) * lda + kb + k);
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int e = t + i * NTH, k = e / BV, col = (e % BV) * VEC;
+        pb[i] = *reinterpret_cast<const CV*>(B + (kb + k) * ldb + ct + col);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        const int e = t + i * NTH, row = rt + e / AV, k = kb + (e % AV) * VEC;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) pa[i].v[j] = (row < m && k + j < K) ? A[row * lda + k + j] : mk<T>(0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int e = t + i * NTH, k = kb + e / BV, col = ct + (e % BV) * VEC;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) pb[i].v[j] = (k < K && col + j < n) ? B[k * ldb + col + j] : mk<T>(0, 0);
+      }
+    }
+  };
+  auto stage = [&](int b) {
+    C* As = L + b * BUF;
+    C* Bs = As + TMT * LDA;
+#pragma unroll
+    for (int i = 0; i < PA; ++i) {
+      const int e = t + i * NTH, row = e / AV, k = (e % AV) * VEC;
+      *reinterpret_cast<CV*>(As + row * LDA + k) = pa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      const int e = t + i * NTH, k = e / BV, col = (e % BV) * VEC;
+      *reinterpret_cast<CV*>(Bs + k * LDB + col) = pb[i];
+    }
+  };
+  fetch(0);
+  stage(0);
+  if (KC < K) fetch(KC);
+  __syncthreads();
+  int b = 0;
+  for (int kb = 0; kb < K; kb += KC) {
+    const C* Ap = L + b * BUF + (wr + li) * LDA + lg;
+    const C* Bp = L + b * BUF + TMT * LDA + lg * LDB + wc + li;
+    const int kend = min(KC, K - kb);  // k in [kend, KC) of the last chunk are staged zeros
+#pragma unroll 2
+    for (int kk = 0; kk < kend; kk += 4) {
+      T ar[RB], ai[RB], as[RB], br[CB], bi[CB], bs[CB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const C a = Ap[rb * 16 * LDA + kk];
+        ar[rb] = a.re;
+        ai[rb] = a.im;
+        as[rb] = a.re + a.im;
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const C bv = Bp[kk * LDB + cb * 16];
+        br[cb] = bv.re;
+        bi[cb] = bv.im;
+        bs[cb] = bv.re + bv.im;
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          x[rb][cb] = Mfma<T>::op(ar[rb], br[cb], x[rb][cb]);
+          y[rb][cb] = Mfma<T>::op(ai[rb], bi[cb], y[rb][cb]);
+          z[rb][cb] = Mfma<T>::op(as[rb], bs[cb], z[rb][cb]);
+        }
+    }
+    // the other buffer was last read in the previous chunk, before the barrier that ended it
+    if (kb + KC < K) {
+      stage(b ^ 1);
+      if (kb + 2 * KC < K) fetch(kb + 2 * KC);
+    }
+    __syncthreads();
+    b ^= 1;
+  }
+  // the last barrier has passed: every wave's reads of A and B are done, so C may alias B
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
+        if (row < m && col < n) {
+          T re = x[rb][cb][v] - y[rb][cb][v], im = (z[rb][cb][v] - x[rb][cb][v]) - y[rb][cb][v];
+          if (!assign) {
+            const C c = Cm[row * ldc + col];
+            re = c.re - re;
+            im = c.im - im;
+          }
+          Cm[row * ldc + col] = mk<T>(re, im);
+        }
+      }
+}
+
 }  // namespace adga
