@@ -60,20 +60,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) k_
   }
 }
 
-// The large Schur updates (mode 0, 128 x 128 tiles) with the 3M product: eight waves, one
-// workgroup per CU (two K-chunk buffers in LDS), the same XCD-aware tile decode as k_gemm.
-constexpr int G3_KC = 32;
-template <typename T, int TMT, int TNT, int WR, int WC>
+// The Schur updates (mode 0) with 64- and 128-wide tiles, by the 3M product: eight waves, two
+// K-chunk buffers in LDS, the same XCD-aware tile decode as k_gemm.
+template <typename T, int TMT, int TNT, int WR, int WC, int KC>
 __global__ void __launch_bounds__(WR * WC * 64) k_gemm3m(int nmat, int ntc, int ntile, long mstride,
                                                          cplx<T>* __restrict__ Mall, int ldm, int r0, int m, int c0,
                                                          int n, int k0, int K) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<T, TMT, TNT, WR, WC, G3_KC>::LDS];
+  __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<T, TMT, TNT, WR, WC, KC>::LDS];
   const int L = blockIdx.x, x = L & 7, sl = L >> 3;
   const int mat = (sl / ntile) * 8 + x, tile = sl % ntile;
   if (mat >= nmat) return;
   const int rt = (tile / ntc) * TMT, ct = (tile % ntc) * TNT;
   cplx<T>* M = Mall + (long)mat * mstride;
-  gemm3m_tile<T, TMT, TNT, WR, WC, G3_KC>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm,
+  gemm3m_tile<T, TMT, TNT, WR, WC, KC>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm,
                                           M + (long)r0 * ldm + c0, ldm, m, n, K, false, rt, ct, lds);
 }
 
@@ -824,6 +823,15 @@ void gemm_launch(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int
                      ldm, r0, m, c0, n, k0, K, mode, inv, istride);
 }
 
+template <typename T, int TMT, int TNT, int WR, int WC, int KC>
+void gemm3m_launch(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int c0, int n, int k0, int K,
+                   hipStream_t s) {
+  const int ntr = (m + TMT - 1) / TMT, ntc = (n + TNT - 1) / TNT, ntile = ntr * ntc;
+  const long nwg = (long)ntile * ((nmat + 7) / 8) * 8;
+  hipLaunchKernelGGL((k_gemm3m<T, TMT, TNT, WR, WC, KC>), dim3((unsigned)nwg), dim3(WR * WC * 64), 0, s, nmat, ntc,
+                     ntile, mstride, M, ldm, r0, m, c0, n, k0, K);
+}
+
 // tile shape by operand shape: the smallest row/column tile (32, 64, 128) covering m / n, so narrow
 // and short updates do not pay for empty MFMA blocks; 128x128 tiles (64x64 per wave) for the large
 // Schur updates halve their re-reads of A and B.  complex128 stops at 128x64 (registers).
@@ -838,16 +846,19 @@ void gemm_region(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int
     timer->mark(s);
     timer->flops += 8.0 * m * (double)n * K * nmat;
   }
-  if constexpr (sizeof(T) == 4) {
-    if (tm == 128 && tn == 128 && mode == 0) {
-      const int ntr = (m + 127) / 128, ntc = (n + 127) / 128, ntile = ntr * ntc;
-      const long nwg = (long)ntile * ((nmat + 7) / 8) * 8;
-      hipLaunchKernelGGL((k_gemm3m<T, 128, 128, 2, 4>), dim3((unsigned)nwg), dim3(512), 0, s, nmat, ntc, ntile,
-                         mstride, M, ldm, r0, m, c0, n, k0, K);
-      if (timed) timer->mark(s);
-      return;
+#define ADGA_G3(TMT, TNT, WR, WC, KC)                                                                         \
+  gemm3m_launch<T, TMT, TNT, WR, WC, KC>(nmat, mstride, M, ldm, r0, m, c0, n, k0, K, s);                     \
+  if (timed) timer->mark(s);                                                                                  \
+  return;
+  if (mode == 0) {
+    if constexpr (sizeof(T) == 4) {
+      if (tm == 128 && tn == 128) { ADGA_G3(128, 128, 2, 4, 32) }
+      if (tm == 128 && tn == 64) { ADGA_G3(128, 64, 4, 2, 16) }
+    } else {
+      if (tm == 128 && tn >= 64) { ADGA_G3(128, 64, 4, 2, 16) }
     }
   }
+#undef ADGA_G3
 #define ADGA_G(TMT, TNT, WR, WC) \
   gemm_launch<T, TMT, TNT, WR, WC>(nmat, mstride, M, ldm, r0, m, c0, n, k0, K, mode, inv, istride, s)
   if (tm == 128 && tn == 128) {

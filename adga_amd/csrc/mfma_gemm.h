@@ -298,8 +298,8 @@ __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda
 // u (|Ar| + |Ai|)(|Br| + |Bi|) instead of u |A||B|; N. J. Higham, SIAM J. Matrix Anal. Appl. 13
 // (1992) 681-687).  Each MFMA takes four consecutive k, lane l supplying k = l/16.  WR x WC waves
 // (the workgroup is WR * WC * 64 threads); two LDS buffers, chunk kb + KC staged into one while
-// chunk kb is multiplied out of the other, so each chunk costs one barrier.  C is read only in
-// the epilogue, after the last barrier.
+// chunk kb is multiplied out of the other, so each chunk costs one barrier.  C is read during
+// the last chunk (the update form only, where C is disjoint from A and B).
 template <typename T, int TMT, int TNT, int WR, int WC, int KC>
 struct G3Cfg {
   static constexpr int NTH = WR * WC * 64;
@@ -386,11 +386,9 @@ __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int l
   stage(0);
   if (KC < K) fetch(KC);
   __syncthreads();
-  int b = 0;
-  for (int kb = 0; kb < K; kb += KC) {
+  auto compute = [&](int b, int kend) {  // k in [kend, KC) of a last, partial chunk are staged zeros
     const C* Ap = L + b * BUF + (wr + li) * LDA + lg;
     const C* Bp = L + b * BUF + TMT * LDA + lg * LDB + wc + li;
-    const int kend = min(KC, K - kb);  // k in [kend, KC) of the last chunk are staged zeros
 #pragma unroll 2
     for (int kk = 0; kk < kend; kk += 4) {
       T ar[RB], ai[RB], as[RB], br[CB], bi[CB], bs[CB];
@@ -417,14 +415,31 @@ __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int l
           z[rb][cb] = Mfma<T>::op(as[rb], bs[cb], z[rb][cb]);
         }
     }
+  };
+  int b = 0, kb = 0;
+  for (; kb + KC < K; kb += KC) {
+    compute(b, KC);
     // the other buffer was last read in the previous chunk, before the barrier that ended it
-    if (kb + KC < K) {
-      stage(b ^ 1);
-      if (kb + 2 * KC < K) fetch(kb + 2 * KC);
-    }
+    stage(b ^ 1);
+    if (kb + 2 * KC < K) fetch(kb + 2 * KC);
     __syncthreads();
     b ^= 1;
   }
+  // the last chunk (no fetch in flight): C of the update form is loaded beside its products
+  C cin[RB][CB][4];
+  if (!assign) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
+          cin[rb][cb][v] = (row < m && col < n) ? Cm[row * ldc + col] : mk<T>(0, 0);
+        }
+  }
+  if (kb < K) compute(b, K - kb);
+  __syncthreads();
   // the last barrier has passed: every wave's reads of A and B are done, so C may alias B
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
@@ -436,9 +451,8 @@ __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int l
         if (row < m && col < n) {
           T re = x[rb][cb][v] - y[rb][cb][v], im = (z[rb][cb][v] - x[rb][cb][v]) - y[rb][cb][v];
           if (!assign) {
-            const C c = Cm[row * ldc + col];
-            re = c.re - re;
-            im = c.im - im;
+            re = cin[rb][cb][v].re - re;
+            im = cin[rb][cb][v].im - im;
           }
           Cm[row * ldc + col] = mk<T>(re, im);
         }
