@@ -263,8 +263,12 @@ def main():
                    "batch_points": args.batch, "precision": "refined", "refine_tol": 1e-12},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["refined"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["refined"], 4),
-                     "traffic": read_traffic("k_gemm_float_128x128_t2g_k20_traffic.md"),
-                     "kernel": "k_gemm<float,128,128,2,2> (batched complex Schur update of the LU, 128x128 tiles)",
+                     "traffic": read_traffic("k_gemm3m_float_128x128_t2g_k20_traffic.md"),
+                     "kernel": "k_gemm3m<float,128,128,2,4,32> (batched complex Schur update of the LU, 128x128 "
+                               "tiles, 3M product)",
+                     # achieved counts 8 flops per complex multiply-add; the 3M product issues 6 of them
+                     # on the MFMA, so the matrix cores' own utilisation is 3/4 of frac
+                     "mfma_work_frac": round(0.75 * achieved / PEAK["refined"], 4),
                      "flops_per_launch": g_avg_flops, "avg_launch_ms": round(g_avg_ms, 4),
                      "launches_timed": ref["gemm_launches"]},
         "lu_factorisation": {"achieved_tflops": round(lu_rate, 3), "avg_ms": round(avg_lu_s * 1e3, 3),
