@@ -157,6 +157,29 @@ def test_gpu_refined_equals_fp64_on_a_batch(t2g_full):
         assert rel_err(res["refined"][k], res["fp64"][k]) < 1e-10, k
 
 
+def test_gpu_complex64_factors_backward_error(t2g_full):
+    """The complex64 LU alone (its Schur updates by the 3M product on 128x128, 128x64, 64x128 and
+    32x128 tiles, N = 1080): with a tolerance the first residual already meets, refinement stops
+    before any correction, so the reported residual max|B - A y| / max|B| is the backward error of
+    the complex64 factors' solve.  Single precision bounds it near c * N * 6e-8; a wrong tile or a
+    dropped term of the 3M combination leaves O(1)."""
+    api = _api()
+    inp = t2g_full
+    pts = [(1, q) for q in range(0, 8000, 83)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    with api.Context(inp, refine_tol=0.5) as ctx:
+        out = api.run_loop(ctx, pts, cl, "refined")
+        st = ctx.stats()
+    assert (out["status"] == 0).all()
+    assert st["refine_iters"] == 0
+    print("complex64 backward error", st["refine_resid"])
+    assert 1e-9 < st["refine_resid"] < 2e-5
+    with api.Context(inp) as ctx:
+        ref = api.run_loop(ctx, pts, cl, "fp64")
+    # one complex64 solve, no correction: chi within single-precision reach of the FP64 path
+    assert rel_err(out["chi_d"], ref["chi_d"]) < 1e-4
+
+
 # (workload, vertex noise, overrides): noise 30 (30000x the default) gives condition numbers
 # 1e3-1e4 at N = 108 and 200 with most rows swapped; at N = 1620 the same noise reaches 1e7, where
 # any two FP64 formulations differ by up to cond * eps ~ 1e-9, so that case uses noise 0.3
