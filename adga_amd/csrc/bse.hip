@@ -100,12 +100,14 @@ void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64
 // (main.F90:693-700, transposed: M = A^T).  A workgroup owns DB frequency blocks d of one channel
 // for PB points and all rows: their S_d sit in LDS, and each thread reads the n2 contiguous values
 // C(d*n2 + ., r) (a row slice of C^T) once and turns them into n2 outputs per point.  Padding
-// columns [N, ldm) are not written (never read).
-template <int NN>
+// columns [N, ldm) are not written (never read).  The complex64 build (the refined path's LU
+// input, whose residuals are formed in FP64 from S and C themselves) computes in FP32: its entries
+// then differ from the rounded FP64 ones by a few FP32 ulps.  The complex128 build computes in FP64.
+template <int NN, int ES>
 struct BuildShape {
   static constexpr int N2 = NN * NN, N4 = N2 * N2;
   static constexpr int DB = NN <= 3 ? 8 : 2;  // frequency blocks per workgroup
-  static constexpr int PB0 = 40960 / (DB * N4 * 16);
+  static constexpr int PB0 = 40960 / (DB * N4 * ES);
   static constexpr int PB = PB0 < 1 ? 1 : PB0 > 4 ? 4 : PB0;  // points per workgroup (share each C row read)
 };
 
@@ -113,9 +115,10 @@ template <typename T, int NN>
 __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const z64* __restrict__ S,
                                                const z64* __restrict__ c_d, const z64* __restrict__ c_m,
                                                cplx<T>* __restrict__ M) {
-  using Sh = BuildShape<NN>;
+  using CA = cplx<T>;  // arithmetic type
+  using Sh = BuildShape<NN, (int)sizeof(CA)>;
   constexpr int N2 = Sh::N2, N4 = Sh::N4, DB = Sh::DB, PB = Sh::PB;
-  __shared__ z64 sS[PB][DB * N4];
+  __shared__ CA sS[PB][DB * N4];
   const int ch = blockIdx.y, b0 = blockIdx.z * PB, t = threadIdx.x;
   const int N = g.N;
   const int d0 = blockIdx.x * DB;
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
   const int np = min(PB, npts - b0);
   for (int pb = 0; pb < np; ++pb) {
     const z64* Sb = S + ((long)(b0 + pb) * g.nf + d0) * N4;
-    for (int e = t; e < nd * N4; e += 256) sS[pb][e] = Sb[e];
+    for (int e = t; e < nd * N4; e += 256) sS[pb][e] = ccast<T>(Sb[e]);
   }
   __syncthreads();
   const z64* C = ch == 0 ? c_d : c_m;
@@ -137,19 +140,19 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
     for (int r0 = 0; r0 < N; r0 += RS) {
       const int r = r0 + rl;
       const bool act = dl < nd && r < N;
-      z64 cc[N2];
+      CA cc[N2];
       const z64* crow = C + (long)(act ? r : 0) * N + (act ? d : 0) * N2;
 #pragma unroll
-      for (int i = 0; i < N2; ++i) cc[i] = crow[i];
+      for (int i = 0; i < N2; ++i) cc[i] = ccast<T>(crow[i]);
       for (int pb = 0; pb < np; ++pb) {
         if (act) {
-          const z64* sd = sS[pb] + dl * N4;
+          const CA* sd = sS[pb] + dl * N4;
 #pragma unroll 1
           for (int sq = 0; sq < N2; ++sq) {
-            z64 acc = mk(0.0, 0.0);
+            CA acc = mk<T>(0, 0);
 #pragma unroll
             for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
-            tile[rl][dl * N2 + sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
+            tile[rl][dl * N2 + sq] = mk<T>((d * N2 + sq == r ? (T)1 : (T)0) - acc.re, -acc.im);
           }
         }
         __syncthreads();
@@ -165,20 +168,20 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
   }
   if (dl >= nd) return;
   for (int r = rl; r < N; r += 256 / DB) {
-    z64 cc[N2];
+    CA cc[N2];
     const z64* crow = C + (long)r * N + d * N2;
 #pragma unroll
-    for (int i = 0; i < N2; ++i) cc[i] = crow[i];
+    for (int i = 0; i < N2; ++i) cc[i] = ccast<T>(crow[i]);
     for (int pb = 0; pb < np; ++pb) {
-      const z64* sd = sS[pb] + dl * N4;
+      const CA* sd = sS[pb] + dl * N4;
       cplx<T>* mrow = M + ((long)(2 * (b0 + pb) + ch) * N + r) * ldm + d * N2;
       // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
 #pragma unroll 1
       for (int sq = 0; sq < N2; ++sq) {
-        z64 acc = mk(0.0, 0.0);
+        CA acc = mk<T>(0, 0);
 #pragma unroll
         for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
-        mrow[sq] = ccast<T>(mk((d * N2 + sq == r ? 1.0 : 0.0) - acc.re, -acc.im));
+        mrow[sq] = mk<T>((d * N2 + sq == r ? (T)1 : (T)0) - acc.re, -acc.im);
       }
     }
   }
@@ -189,7 +192,8 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
                   hipStream_t s) {
 #define ADGA_BUILD_CASE(NN)                                                                            \
   case NN: {                                                                                           \
-    constexpr int DB = BuildShape<NN>::DB, PB = BuildShape<NN>::PB;                                   \
+    using Sh = BuildShape<NN, (int)sizeof(cplx<T>)>;                                                   \
+    constexpr int DB = Sh::DB, PB = Sh::PB;                                                           \
     dim3 grid((g.nf + DB - 1) / DB, 2, (npts + PB - 1) / PB);                                         \
     hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, npts, ldm, S, cd, cm, M);          \
     break;                                                                                             \
