@@ -156,10 +156,26 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
           }
         }
         __syncthreads();
+        // 16-byte stores: ldm is a multiple of 16 and d0 * n2 is even, so every row segment starts
+        // 16-byte aligned
         cplx<T>* mb = M + ((long)(2 * (b0 + pb) + ch) * N + r0) * ldm + d0 * N2;
-        for (int e = t; e < RS * W; e += 256) {
-          const int row = e / W, col = e % W;
-          if (r0 + row < N && col < nd * N2) mb[(long)row * ldm + col] = tile[row][col];
+        constexpr int VW = 16 / (int)sizeof(cplx<T>), WV = W / VW;
+        static_assert(W % VW == 0, "row segment of whole 16-byte vectors");
+        struct alignas(16) Vec { cplx<T> v[VW]; };
+        const int lim = nd * N2;
+        for (int e = t; e < RS * WV; e += 256) {
+          const int row = e / WV, col = (e % WV) * VW;
+          if (r0 + row >= N) continue;
+          cplx<T>* dst = mb + (long)row * ldm + col;
+          if (col + VW <= lim) {
+            Vec x;
+#pragma unroll
+            for (int j = 0; j < VW; ++j) x.v[j] = tile[row][col + j];
+            *reinterpret_cast<Vec*>(dst) = x;
+          } else {
+            for (int j = 0; j < VW; ++j)
+              if (col + j < lim) dst[j] = tile[row][col + j];
+          }
         }
         __syncthreads();
       }
