@@ -228,65 +228,71 @@ template void launch_build<float>(const Geom&, int, int, const z64*, const z64*,
 template void launch_build<double>(const Geom&, int, int, const z64*, const z64*, const z64*, z64*, hipStream_t);
 
 // ---------------------------------------------------------------- A7/A8 refinement residuals
-// Z[m][d*n2 + i'][s] = sum_c' S_b[d][c'][i'] V[m][d*n2 + c'][s]   (m = 2b + ch), i.e. Z = blockdiag(S)^T V,
-// with V = Y (a solution) or, when Y is null, V = (1 + gamma^w_ch)^T.
-__global__ void k_resid_z(Geom g, int nmat, const z64* __restrict__ S, const z64* __restrict__ Y,
+// The residual products take their right operand row-major, one N x (npts n2) plane per channel:
+// Z_ch[x][b*n2 + s] = sum_c' S_b[d][c'][i'] V[2b + ch][d*n2 + c'][s]  (x = d*n2 + i'), i.e.
+// Z = blockdiag(S)^T V with the points side by side, V = Y (a solution) or, when Y is null,
+// V = (1 + gamma^w_ch)^T.  Threads run along a plane row, so the writes are coalesced.
+__global__ void k_resid_z(Geom g, int npts, const z64* __restrict__ S, const z64* __restrict__ Y,
                           const z64* __restrict__ opg_d, const z64* __restrict__ opg_m, z64* __restrict__ Z) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long tot = (long)nmat * g.N * g.n2;
-  if (idx >= tot) return;
-  int s = (int)(idx % g.n2);
-  long mx = idx / g.n2;
-  int x = (int)(mx % g.N);
-  int m = (int)(mx / g.N);
-  int b = m / 2;
-  int d = x / g.n2, ip = x % g.n2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ncol = (long)npts * g.n2;
+  if (idx >= 2 * g.N * ncol) return;
+  const int col = (int)(idx % ncol);
+  const long cx = idx / ncol;
+  const int x = (int)(cx % g.N), ch = (int)(cx / g.N);
+  const int b = col / g.n2, s = col - b * g.n2, m = 2 * b + ch;
+  const int d = x / g.n2, ip = x % g.n2;
   const z64* Sd = S + ((long)b * g.nf + d) * g.n4;
   z64 acc = mk(0.0, 0.0);
   if (Y) {
     const z64* Ym = Y + (long)m * g.N * g.n2;
     for (int cp = 0; cp < g.n2; ++cp) acc = cfma(Sd[cp * g.n2 + ip], Ym[(long)(d * g.n2 + cp) * g.n2 + s], acc);
   } else {
-    const z64* opg = (m % 2 == 0) ? opg_d : opg_m;
+    const z64* opg = ch == 0 ? opg_d : opg_m;
     for (int cp = 0; cp < g.n2; ++cp) acc = cfma(Sd[cp * g.n2 + ip], opg[s + (long)(d * g.n2 + cp) * g.n2], acc);
   }
   Z[idx] = acc;
 }
 
-// R_ch = B - Y + C_ch^T Z for every point of the batch (B, Y optional), on FP64 MFMA: one GEMM per
-// channel, (N x N) C_ch^T times the batch's N x (npts n2) right-hand sides, 128x64 tiles; Z, B, Y, R
-// in the per-matrix layout [(2b + ch) N + row] n2 + s.
-__global__ void __launch_bounds__(GEMM_NT) __attribute__((amdgpu_waves_per_eu(2)))
+// R = B - Y elementwise (the residual product then adds C^T Z into R)
+__global__ void k_sub(long tot, const z64* __restrict__ B, const z64* __restrict__ Y, z64* __restrict__ R) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < tot) R[i] = csub(B[i], Y[i]);
+}
+
+// Out[(2b + ch) N + r][s] (+)= sum_x C_ch(x, r) Z_ch[x][b*n2 + s]: the FP64 residual product on the
+// 3M tile (128 x 64, eight waves), C read row-major as A(r, x) = C[r N + x], the output through the
+// point-major layout of Y.  accumulate = 0 writes the product, 1 adds it to what Out holds.
+// Workgroups are dealt round-robin to the 8 XCDs; the decode puts all row tiles of one column tile
+// on one XCD, adjacent in time, so that the column block of Z (N x 64) is shared through its L2.
+constexpr int RG_TM = 128, RG_TN = 64, RG_WR = 4, RG_WC = 2, RG_KC = 16;
+__global__ void __launch_bounds__(RG_WR * RG_WC * 64)
 k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* __restrict__ cm,
-             const z64* __restrict__ B, const z64* __restrict__ Y, const z64* __restrict__ Z, z64* __restrict__ R) {
-  constexpr int TMT = 128, TNT = 64;
-  __shared__ __attribute__((aligned(16))) unsigned char lds[gemm_lds_bytes<double>()];
+             const z64* __restrict__ Z, z64* __restrict__ Out, int accumulate) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<double, RG_TM, RG_TN, RG_WR, RG_WC, RG_KC>::LDS];
   const int ch = ch0 + (int)blockIdx.y;
   const z64* C = ch == 0 ? cd : cm;
   const int N = g.N, n2 = g.n2, ncol = npts * n2;
-  const int ntc = (ncol + TNT - 1) / TNT;
-  const int rt = ((int)blockIdx.x / ntc) * TMT, ct = ((int)blockIdx.x % ntc) * TNT;
-  gemm_tile_acc<double, TMT, TNT, 2, 2>(
-      [=](int r, int x) { return C[(long)r * N + x]; },  // C^T(r, x) = C(x, r)
-      [=](int x, int col) {
-        const int b = col / n2, sq = col - b * n2;
-        return Z[((long)(2 * b + ch) * N + x) * n2 + sq];
-      },
-      [=](int, int) { return mk(0.0, 0.0); },
-      [=](int r, int col, z64 v) {
-        const int b = col / n2, sq = col - b * n2;
-        const long yi = ((long)(2 * b + ch) * N + r) * n2 + sq;
-        z64 o = B ? B[yi] : mk(0.0, 0.0);
-        if (Y) o = csub(o, Y[yi]);
-        R[yi] = cadd(o, v);
-      },
-      N, ncol, N, true, rt, ct, lds);
+  const int ntr = (N + RG_TM - 1) / RG_TM, ntc = (ncol + RG_TN - 1) / RG_TN;
+  const int L = (int)blockIdx.x, xcd = L & 7, j = L >> 3;
+  const int tc = (j / ntr) * 8 + xcd, tr = j % ntr;
+  if (tc >= ntc) return;
+  const int rt = tr * RG_TM, ct = tc * RG_TN;
+  auto at = [=](int r, int col) {
+    const int b = col / n2, sq = col - b * n2;
+    return ((long)(2 * b + ch) * N + r) * n2 + sq;
+  };
+  gemm3m_tile<double, RG_TM, RG_TN, RG_WR, RG_WC, RG_KC>(
+      C, N, Z + (long)ch * N * ncol, ncol, [=](int r, int col) { return Out[at(r, col)]; },
+      [=](int r, int col, z64 v) { Out[at(r, col)] = v; }, N, ncol, N, accumulate == 0, false, rt, ct, lds);
 }
 
-static void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* B,
-                       const z64* Y, const z64* Z, z64* R, hipStream_t s) {
-  const int ntr = (g.N + 127) / 128, ntc = (npts * g.n2 + 63) / 64;
-  hipLaunchKernelGGL(k_resid_gemm, dim3(ntr * ntc, nch), dim3(GEMM_NT), 0, s, g, npts, ch0, cd, cm, B, Y, Z, R);
+static void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* Z,
+                       z64* Out, bool accumulate, hipStream_t s) {
+  const int ntr = (g.N + RG_TM - 1) / RG_TM, ntc = (npts * g.n2 + RG_TN - 1) / RG_TN;
+  const int nwg = ntr * ((ntc + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_resid_gemm, dim3(nwg, nch), dim3(RG_WR * RG_WC * 64), 0, s, g, npts, ch0, cd, cm, Z, Out,
+                     accumulate ? 1 : 0);
 }
 
 // Per-matrix max |R| / max |B| (|.| = |re|+|im|), one workgroup per matrix; a zero right-hand
@@ -313,26 +319,25 @@ __global__ void __launch_bounds__(256) k_resid_norm(Geom g, const z64* __restric
   if (t == 0) rel[m] = sb[0] > 0.0 ? sr[0] / sb[0] : (sr[0] > 0.0 ? INFINITY : 0.0);
 }
 
-// Zg[2b][d*n2 + i'][s] = S_b[d][s][i'], so that C^T Zg (k_resid_gemm) is
+// Zg[d*n2 + i'][b*n2 + s] = S_b[d][s][i'] (the channel-0 plane of the residual product's right
+// operand), so that C^T Zg (k_resid_gemm) is
 // gamma^q_d(s, r) = sum_d sum_i' S_b[d][s][i'] C_d(d*n2 + i', r) = sum_nu chi0^{nl,q} F^w_d
 // (main.F90:693-700, the accumulation of rectanglework over nu).
 __global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __restrict__ Z) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long tot = (long)npts * g.N * g.n2;
-  if (idx >= tot) return;
-  int s = (int)(idx % g.n2);
-  long bx = idx / g.n2;
-  int x = (int)(bx % g.N);
-  int b = (int)(bx / g.N);
-  int d = x / g.n2, ip = x % g.n2;
-  Z[((long)(2 * b) * g.N + x) * g.n2 + s] = S[((long)b * g.nf + d) * g.n4 + s * g.n2 + ip];
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ncol = (long)npts * g.n2;
+  if (idx >= g.N * ncol) return;
+  const int col = (int)(idx % ncol), x = (int)(idx / ncol);
+  const int b = col / g.n2, s = col - b * g.n2;
+  const int d = x / g.n2, ip = x % g.n2;
+  Z[idx] = S[((long)b * g.nf + d) * g.n4 + s * g.n2 + ip];
 }
 
 // gamma^q_d for every point of the batch, into G2 with the residual layout (point b at 2*b).
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
   long tot = (long)npts * g.N * g.n2;
   hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
-  resid_gemm(g, npts, 0, 1, cd, cd, nullptr, nullptr, Z, G2, s);
+  resid_gemm(g, npts, 0, 1, cd, cd, Z, G2, false, s);
 }
 
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.
@@ -341,9 +346,9 @@ void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64*
                 const z64* opgm, z64* Z, z64* B, hipStream_t s) {
   int nmat = 2 * npts;
   long tot = (long)nmat * g.N * g.n2;
-  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nmat, S, nullptr, opgd,
+  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, nullptr, opgd,
                      opgm, Z);
-  resid_gemm(g, npts, 0, 2, cd, cm, nullptr, nullptr, Z, B, s);
+  resid_gemm(g, npts, 0, 2, cd, cm, Z, B, false, s);
 }
 
 // R = B - M Y with M Y = Y - C^T (blockdiag(S)^T Y), in FP64 from the unrounded factors S and C.
@@ -351,9 +356,10 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
                      const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s) {
   int nmat = 2 * npts;
   long tot = (long)nmat * g.N * g.n2;
-  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nmat, S, Y, nullptr,
+  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Y, nullptr,
                      nullptr, Z);
-  resid_gemm(g, npts, 0, 2, cd, cm, B, Y, Z, R, s);
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, B, Y, R);
+  resid_gemm(g, npts, 0, 2, cd, cm, Z, R, true, s);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);
 }
 

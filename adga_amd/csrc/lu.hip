@@ -72,8 +72,10 @@ __global__ void __launch_bounds__(WR * WC * 64) k_gemm3m(int nmat, int ntc, int 
   if (mat >= nmat) return;
   const int rt = (tile / ntc) * TMT, ct = (tile % ntc) * TNT;
   cplx<T>* M = Mall + (long)mat * mstride;
-  gemm3m_tile<T, TMT, TNT, WR, WC, KC>(M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm,
-                                          M + (long)r0 * ldm + c0, ldm, m, n, K, false, rt, ct, lds);
+  cplx<T>* Cp = M + (long)r0 * ldm + c0;
+  gemm3m_tile<T, TMT, TNT, WR, WC, KC>(
+      M + (long)r0 * ldm + k0, ldm, M + (long)k0 * ldm + c0, ldm, [=](int r, int c) { return Cp[r * ldm + c]; },
+      [=](int r, int c, cplx<T> v) { Cp[r * ldm + c] = v; }, m, n, K, false, true, rt, ct, lds);
 }
 
 // component-wise select of two complex values (a select of whole structs can turn into a select of

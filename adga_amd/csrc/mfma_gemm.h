@@ -292,8 +292,9 @@ __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda
 
 
 // ---------------------------------------------------------------- 3M tile (three real products)
-// The row-major product of gemm_tile with the complex product formed from three real products on
-// the MFMA instead of four: X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi), Re(AB) = X - Y,
+// C[0:m, 0:n] <- C -/+ A B (assign = false, sub = true/false) or -/+ A B (assign = true) for
+// row-major A and B, C read through cload(row, col) and written through cstore(row, col, value),
+// with the complex product formed from three real products on the MFMA instead of four: X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi), Re(AB) = X - Y,
 // Im(AB) = Z - X - Y (the 3M method: normwise stable, the imaginary part's error bounded by
 // u (|Ar| + |Ai|)(|Br| + |Bi|) instead of u |A||B|; N. J. Higham, SIAM J. Matrix Anal. Appl. 13
 // (1992) 681-687).  Each MFMA takes four consecutive k, lane l supplying k = l/16.  WR x WC waves
@@ -311,10 +312,10 @@ struct G3Cfg {
   static constexpr int LDS = 2 * BUF * CS;
 };
 
-template <typename T, int TMT, int TNT, int WR, int WC, int KC>
+template <typename T, int TMT, int TNT, int WR, int WC, int KC, class CL, class CS>
 __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int lda, const cplx<T>* B, int ldb,
-                                            cplx<T>* Cm, int ldc, int m, int n, int K, bool assign, int rt, int ct,
-                                            unsigned char* lds) {
+                                            CL cload, CS cstore, int m, int n, int K, bool assign, bool sub,
+                                            int rt, int ct, unsigned char* lds) {
   using V = typename Mfma<T>::V;
   using C = cplx<T>;
   using Cfg = G3Cfg<T, TMT, TNT, WR, WC, KC>;
@@ -435,7 +436,7 @@ __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int l
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
-          cin[rb][cb][v] = (row < m && col < n) ? Cm[row * ldc + col] : mk<T>(0, 0);
+          cin[rb][cb][v] = (row < m && col < n) ? cload(row, col) : mk<T>(0, 0);
         }
   }
   if (kb < K) compute(b, K - kb);
@@ -450,11 +451,15 @@ __device__ __forceinline__ void gemm3m_tile(const cplx<T>* __restrict__ A, int l
         const int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
         if (row < m && col < n) {
           T re = x[rb][cb][v] - y[rb][cb][v], im = (z[rb][cb][v] - x[rb][cb][v]) - y[rb][cb][v];
-          if (!assign) {
-            re = cin[rb][cb][v].re - re;
-            im = cin[rb][cb][v].im - im;
+          if (sub) {
+            re = -re;
+            im = -im;
           }
-          Cm[row * ldc + col] = mk<T>(re, im);
+          if (!assign) {
+            re = cin[rb][cb][v].re + re;
+            im = cin[rb][cb][v].im + im;
+          }
+          cstore(row, col, mk<T>(re, im));
         }
       }
 }
