@@ -1,8 +1,9 @@
-// Complex GEMM tile on gfx950 MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64), shared by the
-// LU's Schur updates and the FP64 BSE residual products.  A complex product is a real product with
-// K doubled and re/im interleaved along it, the lane's K slot selecting (re, -im) or (im, re) of the
-// A element and (re, im) of the B element.  Operands are reached through accessor functors, so the
-// same tile serves row-major matrices and the batched point layouts.
+// Complex GEMM tiles on gfx950 MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64) for the LU's
+// Schur updates and triangular products and the FP64 BSE residual products.  gemm_tile forms a
+// complex product as a real product with K doubled and re/im interleaved along it (the lane's K
+// slot selecting (re, -im) or (im, re) of the A element and (re, im) of the B element);
+// gemm3m_tile forms it from three real products (the 3M method), with C reached through accessor
+// functors so that the same tile serves row-major matrices and the batched point layouts.
 #pragma once
 
 #include "adga_common.h"
@@ -36,124 +37,13 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int lg, int v) { return lg + 4 * v; }
 };
 
-// LDS staging of one GEMM K chunk: A transposed At[k][row], B as Bs[k][col]
-template <typename T>
-constexpr int gemm_lds_bytes() { return 16 * (128 + 4 + 128 + 4) * 2 * (int)sizeof(T); }
-
-// ---------------------------------------------------------------- GEMM tile on MFMA
-// One TMT x TNT tile (rows rt.., columns ct..) of C[0:m, 0:n] <- C - A B (assign = false) or A B
-// (assign = true), A[0:m, 0:K], B[0:K, 0:n] read through aload(row, k) / bload(k, col), C through
-// cload(row, col) / cstore(row, col, value); the four waves in a WR x WC grid, each
-// a (TMT/WR) x (TNT/WC) block of 16x16 MFMA tiles.  K is staged through LDS in chunks of KC; the
-// next chunk's global loads are issued into registers before the current chunk's MFMAs, so load
-// latency overlaps the matrix work.  The output is written after all of K has been read by every
-// wave, so C may alias B when the tile spans all K rows of B (in-place solves: m = K <= TMT).
-template <typename T, int TMT, int TNT, int WR, int WC, class AL, class BL, class CL, class CS>
-__device__ __forceinline__ void gemm_tile_acc(AL aload, BL bload, CL cload, CS cstore, int m, int n, int K, bool assign,
-                                              int rt, int ct, unsigned char* lds) {
-  using V = typename Mfma<T>::V;
-  using C = cplx<T>;
-  static_assert(WR * WC == GEMM_NT / 64, "four waves");
-  constexpr int WTM = TMT / WR, WTN = TNT / WC, RB = WTM / 16, CB = WTN / 16;
-  static_assert(RB >= 1 && CB >= 1 && WTM % 16 == 0 && WTN % 16 == 0, "wave tile of 16x16 blocks");
-  constexpr int KC0 = gemm_lds_bytes<T>() / ((TMT + 4 + TNT + 4) * (int)sizeof(C)) / 8 * 8;
-  constexpr int KC = KC0 > 32 ? 32 : KC0;
-  static_assert(KC >= 8 && (TMT * KC) % GEMM_NT == 0 && (KC * TNT) % GEMM_NT == 0, "K chunk");
-  constexpr int PA = TMT * KC / GEMM_NT, PB = KC * TNT / GEMM_NT;  // prefetched elements per thread
-  C(&At)[KC][TMT + 4] = *reinterpret_cast<C(*)[KC][TMT + 4]>(lds);
-  C(&Bs)[KC][TNT + 4] = *reinterpret_cast<C(*)[KC][TNT + 4]>(lds + sizeof(C) * KC * (TMT + 4));
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wr = (wave / WC) * WTM, wc = (wave % WC) * WTN;
-  const int li = lane & 15, lg = lane >> 4, part = lg & 1, kcl = lg >> 1;
-  const T sgn = assign ? (T)1 : (T)-1;
-  V accr[RB][CB], acci[RB][CB];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
-        C c = (!assign && row < m && col < n) ? cload(row, col) : mk<T>(0, 0);
-        accr[rb][cb][v] = c.re;
-        acci[rb][cb][v] = c.im;
-      }
-  C pa[PA], pb[PB];
-#pragma unroll
-  for (int i = 0; i < PA; ++i) {
-    int e = t + i * GEMM_NT, row = rt + e / KC, k = e % KC;
-    pa[i] = (row < m && k < K) ? aload(row, k) : mk<T>(0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < PB; ++i) {
-    int e = t + i * GEMM_NT, k = e / TNT, col = ct + e % TNT;
-    pb[i] = (k < K && col < n) ? bload(k, col) : mk<T>(0, 0);
-  }
-  for (int kb = 0; kb < K; kb += KC) {
-    if (kb) __syncthreads();  // the previous chunk's MFMAs are done with LDS
-#pragma unroll
-    for (int i = 0; i < PA; ++i) {
-      int e = t + i * GEMM_NT;
-      At[e % KC][e / KC] = pa[i];
-    }
-#pragma unroll
-    for (int i = 0; i < PB; ++i) {
-      int e = t + i * GEMM_NT;
-      Bs[e / TNT][e % TNT] = pb[i];
-    }
-    __syncthreads();
-    const int kn = kb + KC;
-    if (kn < K) {
-#pragma unroll
-      for (int i = 0; i < PA; ++i) {
-        int e = t + i * GEMM_NT, row = rt + e / KC, k = kn + e % KC;
-        pa[i] = (row < m && k < K) ? aload(row, k) : mk<T>(0, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < PB; ++i) {
-        int e = t + i * GEMM_NT, k = kn + e / TNT, col = ct + e % TNT;
-        pb[i] = (k < K && col < n) ? bload(k, col) : mk<T>(0, 0);
-      }
-    }
-    const int kend = min(KC, K - kb);
-#pragma unroll 2
-    for (int kk = 0; kk < kend; kk += 2) {
-      T ar[RB], ai[RB], bb[CB];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        C a = At[kk + kcl][wr + rb * 16 + li];
-        ar[rb] = sgn * (part ? -a.im : a.re);
-        ai[rb] = sgn * (part ? a.re : a.im);
-      }
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        C b = Bs[kk + kcl][wc + cb * 16 + li];
-        bb[cb] = part ? b.im : b.re;
-      }
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-          accr[rb][cb] = Mfma<T>::op(ar[rb], bb[cb], accr[rb][cb]);
-          acci[rb][cb] = Mfma<T>::op(ai[rb], bb[cb], acci[rb][cb]);
-        }
-    }
-  }
-  __syncthreads();  // every wave has consumed its operands before the output (may alias B) lands
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int row = rt + wr + rb * 16 + Mfma<T>::row(lg, v), col = ct + wc + cb * 16 + li;
-        if (row < m && col < n) cstore(row, col, mk<T>(accr[rb][cb][v], acci[rb][cb][v]));
-      }
-}
-
-
 // ---------------------------------------------------------------- row-major tile (the LU's GEMMs)
-// Same product as gemm_tile_acc for row-major A, B, C, with the staging tuned for that case:
+// C[0:m, 0:n] <- C - A B (assign = false) or A B (assign = true) for row-major A, B, C (the LU's
+// updates and triangular products), the four waves in a WR x WC grid, each a (TMT/WR) x (TNT/WC)
+// block of 16x16 MFMA tiles; K staged through LDS in chunks of KC, the next chunk's global loads
+// issued into registers before the current chunk's MFMAs; the output is written after all of K
+// has been read by every wave, so C may alias B when the tile spans all K rows of B (in-place
+// triangular products: m = K <= TMT).  Staging:
 // 16-byte global loads (two complex64 or one complex128 per load) without bounds checks on
 // interior tiles of aligned operands, int offsets, and LDS layouts read without bank conflicts by
 // the MFMA fragments: A as As[row][k] (row stride KC complex + 16 bytes), B as Bs[k][col] (row
