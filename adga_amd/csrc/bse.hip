@@ -231,16 +231,17 @@ template void launch_build<double>(const Geom&, int, int, const z64*, const z64*
 // The residual products take their right operand row-major, one N x (npts n2) plane per channel:
 // Z_ch[x][b*n2 + s] = sum_c' S_b[d][c'][i'] V[2b + ch][d*n2 + c'][s]  (x = d*n2 + i'), i.e.
 // Z = blockdiag(S)^T V with the points side by side, V = Y (a solution) or, when Y is null,
-// V = (1 + gamma^w_ch)^T.  Threads run along a plane row, so the writes are coalesced.
+// V = (1 + gamma^w_ch)^T.  Threads run over one matrix m = 2b + ch at a time (x, then s fastest),
+// so the reads of S_b and V_m stay local; the writes land in runs of n2 values.
 __global__ void k_resid_z(Geom g, int npts, const z64* __restrict__ S, const z64* __restrict__ Y,
                           const z64* __restrict__ opg_d, const z64* __restrict__ opg_m, z64* __restrict__ Z) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ncol = (long)npts * g.n2;
   if (idx >= 2 * g.N * ncol) return;
-  const int col = (int)(idx % ncol);
-  const long cx = idx / ncol;
-  const int x = (int)(cx % g.N), ch = (int)(cx / g.N);
-  const int b = col / g.n2, s = col - b * g.n2, m = 2 * b + ch;
+  const int s = (int)(idx % g.n2);
+  const long mx = idx / g.n2;
+  const int x = (int)(mx % g.N), m = (int)(mx / g.N);
+  const int b = m / 2, ch = m % 2;
   const int d = x / g.n2, ip = x % g.n2;
   const z64* Sd = S + ((long)b * g.nf + d) * g.n4;
   z64 acc = mk(0.0, 0.0);
@@ -251,7 +252,7 @@ __global__ void k_resid_z(Geom g, int npts, const z64* __restrict__ S, const z64
     const z64* opg = ch == 0 ? opg_d : opg_m;
     for (int cp = 0; cp < g.n2; ++cp) acc = cfma(Sd[cp * g.n2 + ip], opg[s + (long)(d * g.n2 + cp) * g.n2], acc);
   }
-  Z[idx] = acc;
+  Z[((long)ch * g.N + x) * ncol + (long)b * g.n2 + s] = acc;
 }
 
 // R = B - Y elementwise (the residual product then adds C^T Z into R)
