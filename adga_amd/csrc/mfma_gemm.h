@@ -184,13 +184,15 @@ __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda
 // ---------------------------------------------------------------- 3M tile (three real products)
 // C[0:m, 0:n] <- C -/+ A B (assign = false, sub = true/false) or -/+ A B (assign = true) for
 // row-major A and B, C read through cload(row, col) and written through cstore(row, col, value),
-// with the complex product formed from three real products on the MFMA instead of four: X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi), Re(AB) = X - Y,
-// Im(AB) = Z - X - Y (the 3M method: normwise stable, the imaginary part's error bounded by
-// u (|Ar| + |Ai|)(|Br| + |Bi|) instead of u |A||B|; N. J. Higham, SIAM J. Matrix Anal. Appl. 13
-// (1992) 681-687).  Each MFMA takes four consecutive k, lane l supplying k = l/16.  WR x WC waves
-// (the workgroup is WR * WC * 64 threads); two LDS buffers, chunk kb + KC staged into one while
-// chunk kb is multiplied out of the other, so each chunk costs one barrier.  C is read during
-// the last chunk (the update form only, where C is disjoint from A and B).
+// with the complex product formed from three real products on the MFMA instead of four:
+// X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi), Re(AB) = X - Y, Im(AB) = Z - X - Y (the 3M method:
+// normwise stable, the imaginary part's error bounded by u (|Ar| + |Ai|)(|Br| + |Bi|) instead of
+// u |A||B|; N. J. Higham, SIAM J. Matrix Anal. Appl. 13 (1992) 681-687).  Each MFMA takes four
+// consecutive k, lane l supplying k = l/16.  WR x WC waves (the workgroup is WR * WC * 64
+// threads); two LDS buffers, chunk kb + KC staged into one while chunk kb is multiplied out of the
+// other, so each chunk costs one barrier.  With assign = false, C is read during the last chunk,
+// so it must not alias A or B; with assign = true, C is written after the last barrier and may
+// alias B.
 template <typename T, int TMT, int TNT, int WR, int WC, int KC>
 struct G3Cfg {
   static constexpr int NTH = WR * WC * 64;
