@@ -192,7 +192,7 @@ __device__ __forceinline__ void gemm_tile(const cplx<T>* __restrict__ A, int lda
 // threads); two LDS buffers, chunk kb + KC staged into one while chunk kb is multiplied out of the
 // other, so each chunk costs one barrier.  With assign = false, C is read during the last chunk,
 // so it must not alias A or B; with assign = true, C is written after the last barrier and may
-// alias B.
+// alias B when the tile spans all K rows of B (m = K <= TMT).
 template <typename T, int TMT, int TNT, int WR, int WC, int KC>
 struct G3Cfg {
   static constexpr int NTH = WR * WC * 64;
