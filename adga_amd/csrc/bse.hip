@@ -255,21 +255,16 @@ __global__ void k_resid_z(Geom g, int npts, const z64* __restrict__ S, const z64
   Z[((long)ch * g.N + x) * ncol + (long)b * g.n2 + s] = acc;
 }
 
-// R = B - Y elementwise (the residual product then adds C^T Z into R)
-__global__ void k_sub(long tot, const z64* __restrict__ B, const z64* __restrict__ Y, z64* __restrict__ R) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < tot) R[i] = csub(B[i], Y[i]);
-}
-
-// Out[(2b + ch) N + r][s] (+)= sum_x C_ch(x, r) Z_ch[x][b*n2 + s]: the FP64 residual product on the
-// 3M tile (128 x 64, eight waves), C read row-major as A(r, x) = C[r N + x], the output through the
-// point-major layout of Y.  accumulate = 0 writes the product, 1 adds it to what Out holds.
-// Workgroups are dealt round-robin to the 8 XCDs; the decode puts all row tiles of one column tile
-// on one XCD, adjacent in time, so that the column block of Z (N x 64) is shared through its L2.
+// Out[(2b + ch) N + r][s] = sum_x C_ch(x, r) Z_ch[x][b*n2 + s], plus (B - Y) at the same position
+// when B is given (the residual): the FP64 products on the 3M tile (128 x 64, eight waves), C read
+// row-major as A(r, x) = C[r N + x], the output in the point-major layout of Y.  Workgroups are
+// dealt round-robin to the 8 XCDs; the decode puts all row tiles of one column tile on one XCD,
+// adjacent in time, so that the column block of Z (N x 64) is shared through its L2.
 constexpr int RG_TM = 128, RG_TN = 64, RG_WR = 4, RG_WC = 2, RG_KC = 16;
 __global__ void __launch_bounds__(RG_WR * RG_WC * 64)
 k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* __restrict__ cm,
-             const z64* __restrict__ Z, z64* __restrict__ Out, int accumulate) {
+             const z64* __restrict__ Z, z64* __restrict__ Out, const z64* __restrict__ B,
+             const z64* __restrict__ Y) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<double, RG_TM, RG_TN, RG_WR, RG_WC, RG_KC>::LDS];
   const int ch = ch0 + (int)blockIdx.y;
   const z64* C = ch == 0 ? cd : cm;
@@ -284,16 +279,20 @@ k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* _
     return ((long)(2 * b + ch) * N + r) * n2 + sq;
   };
   gemm3m_tile<double, RG_TM, RG_TN, RG_WR, RG_WC, RG_KC>(
-      C, N, Z + (long)ch * N * ncol, ncol, [=](int r, int col) { return Out[at(r, col)]; },
-      [=](int r, int col, z64 v) { Out[at(r, col)] = v; }, N, ncol, N, accumulate == 0, false, rt, ct, lds);
+      C, N, Z + (long)ch * N * ncol, ncol,
+      [=](int r, int col) {
+        const long i = at(r, col);
+        return csub(B[i], Y[i]);
+      },
+      [=](int r, int col, z64 v) { Out[at(r, col)] = v; }, N, ncol, N, B == nullptr, false, rt, ct, lds);
 }
 
 static void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* Z,
-                       z64* Out, bool accumulate, hipStream_t s) {
+                       z64* Out, const z64* B, const z64* Y, hipStream_t s) {
   const int ntr = (g.N + RG_TM - 1) / RG_TM, ntc = (npts * g.n2 + RG_TN - 1) / RG_TN;
   const int nwg = ntr * ((ntc + 7) / 8) * 8;
-  hipLaunchKernelGGL(k_resid_gemm, dim3(nwg, nch), dim3(RG_WR * RG_WC * 64), 0, s, g, npts, ch0, cd, cm, Z, Out,
-                     accumulate ? 1 : 0);
+  hipLaunchKernelGGL(k_resid_gemm, dim3(nwg, nch), dim3(RG_WR * RG_WC * 64), 0, s, g, npts, ch0, cd, cm, Z, Out, B,
+                     Y);
 }
 
 // Per-matrix max |R| / max |B| (|.| = |re|+|im|), one workgroup per matrix; a zero right-hand
@@ -338,7 +337,7 @@ __global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __res
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
   long tot = (long)npts * g.N * g.n2;
   hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
-  resid_gemm(g, npts, 0, 1, cd, cd, Z, G2, false, s);
+  resid_gemm(g, npts, 0, 1, cd, cd, Z, G2, nullptr, nullptr, s);
 }
 
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.
@@ -349,7 +348,7 @@ void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64*
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, nullptr, opgd,
                      opgm, Z);
-  resid_gemm(g, npts, 0, 2, cd, cm, Z, B, false, s);
+  resid_gemm(g, npts, 0, 2, cd, cm, Z, B, nullptr, nullptr, s);
 }
 
 // R = B - M Y with M Y = Y - C^T (blockdiag(S)^T Y), in FP64 from the unrounded factors S and C.
@@ -359,8 +358,7 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Y, nullptr,
                      nullptr, Z);
-  hipLaunchKernelGGL(k_sub, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, B, Y, R);
-  resid_gemm(g, npts, 0, 2, cd, cm, Z, R, true, s);
+  resid_gemm(g, npts, 0, 2, cd, cm, Z, R, B, Y, s);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);
 }
 

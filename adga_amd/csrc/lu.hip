@@ -854,7 +854,7 @@ void gemm_region(int nmat, long mstride, cplx<T>* M, int ldm, int r0, int m, int
   return;
   if (mode == 0) {
     if constexpr (sizeof(T) == 4) {
-      if (tm == 128 && tn == 128) { ADGA_G3(128, 128, 2, 4, 32) }
+      if (tm == 128 && tn == 128) { ADGA_G3(128, 128, 4, 2, 32) }
       if (tm == 128 && tn == 64) { ADGA_G3(128, 64, 4, 2, 16) }
       if (tm == 64 && tn == 128) { ADGA_G3(64, 128, 2, 4, 16) }
       if (tm == 32 && tn == 128) { ADGA_G3(32, 128, 1, 4, 16) }

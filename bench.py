@@ -264,7 +264,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["refined"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["refined"], 4),
                      "traffic": read_traffic("k_gemm3m_float_128x128_t2g_k20_traffic.md"),
-                     "kernel": "k_gemm3m<float,128,128,2,4,32> (batched complex Schur update of the LU, 128x128 "
+                     "kernel": "k_gemm3m<float,128,128,4,2,32> (batched complex Schur update of the LU, 128x128 "
                                "tiles, 3M product)",
                      # achieved counts 8 flops per complex multiply-add; the 3M product issues 6 of them
                      # on the MFMA, so the matrix cores' own utilisation is 3/4 of frac
