@@ -647,14 +647,24 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
             k += 2 * G;
           }
         }
-        for (; k < k1; k += 16) {  // the rest: fewer than G columns
-          const C a0 = La[k], a1 = Lb[k];
-          const C* xr = x + k * CG;
+        if (k < k1) {  // the rest, fewer than G columns: one predicated group, its loads in flight together
 #pragma unroll
-          for (int s = 0; s < CG; ++s) {
-            const C xv = xr[s];
-            acc0[s] = cfma(a0, xv, acc0[s]);
-            acc1[s] = cfma(a1, xv, acc1[s]);
+          for (int u = 0; u < SU; ++u) {
+            const bool in = k + 16 * u < k1;
+            pa0[u] = in ? La[k + 16 * u] : mk<T>(0, 0);
+            pa1[u] = in ? Lb[k + 16 * u] : mk<T>(0, 0);
+          }
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            if (k + 16 * u < k1) {
+              const C* xr = x + (k + 16 * u) * CG;
+#pragma unroll
+              for (int s = 0; s < CG; ++s) {
+                const C xv = xr[s];
+                acc0[s] = cfma(pa0[u], xv, acc0[s]);
+                acc1[s] = cfma(pa1[u], xv, acc1[s]);
+              }
+            }
           }
         }
 #pragma unroll
