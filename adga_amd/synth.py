@@ -50,6 +50,9 @@ class Workload:
     nq: tuple | None = None     # q grid (defaults to the k grid)
     iwmax: int | None = None    # 1P frequencies (defaults to 4*(Nf/2+Nw/2))
     noise: float = 1e-3
+    k_eom: str = "all"          # EOM k list: "all" (the grid in order) or "sub" (every third k, reversed)
+    q_list: str = "vol"         # q list: "vol" (generate_q_vol on qgrid) or "path" (a path through the grid)
+    vq: float = 0.0             # scale of the synthetic non-local interaction V(q) (0: no V(q))
 
     @property
     def nkp(self):
@@ -61,8 +64,15 @@ class Workload:
 
     @property
     def nqp(self):
+        if self.q_list == "path":
+            return len(q_path(self.nk))
         q = self.qgrid
         return q[0] * q[1] * q[2]
+
+    @property
+    def do_eom(self):
+        # the reference stops when a q path is combined with the EOM (main.F90:341-347)
+        return self.q_list != "path"
 
     @property
     def nf(self):
@@ -92,6 +102,17 @@ WORKLOADS = {
     "t2g_k12": Workload("t2g_k12", 3, (12, 12, 12), 40, 10, 10.0, 5.0, 0.75, "t2g",
                         t=-0.26, tp=-0.08, seed=20261206),
     # Small cases used by the parity tests (each finishes in seconds on the CPU).
+    # EOM on a reordered third of the k grid with q on a 2x2x2 sub-grid (k_path_eom + q volume)
+    "t2g_k4_keom": Workload("t2g_k4_keom", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
+                            t=-0.26, tp=-0.08, seed=16, nq=(2, 2, 2), k_eom="sub"),
+    # susceptibility along a q path (no EOM, as the reference requires)
+    "t2g_k6_qpath": Workload("t2g_k6_qpath", 3, (6, 6, 6), 6, 2, 10.0, 5.0, 0.75, "t2g",
+                             t=-0.26, tp=-0.08, seed=17, q_list="path"),
+    # non-local interaction V(q): density-channel BSE term, Hartree/Fock, V terms of the EOM
+    "two_k4_vq": Workload("two_k4_vq", 2, (4, 4, 2), 4, 2, 8.0, 3.0, 0.5, "rand5",
+                          t=0.3, seed=18, vq=0.4),
+    "t2g_k4_vq": Workload("t2g_k4_vq", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
+                          t=-0.26, tp=-0.08, seed=19, vq=0.5, k_eom="sub"),
     "t2g_k4": Workload("t2g_k4", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
                        t=-0.26, tp=-0.08, seed=11),
     "t2g_k6": Workload("t2g_k6", 3, (6, 6, 6), 12, 3, 10.0, 5.0, 0.75, "t2g",
@@ -126,6 +147,7 @@ class Inputs:
     n_dmft: np.ndarray    # (n,) complex
     q_data: np.ndarray    # (nqp,) 1-based k indices of the q points
     k_data_eom: np.ndarray  # (nkp_eom,) 1-based k indices of the EOM k points
+    v_q: np.ndarray | None = None  # (n^2, n^2, nqp) V(q) in read_vq's compound index, or None
 
     @property
     def n(self):
@@ -236,6 +258,41 @@ def k_index(kx, ky, kz, nk):
     return 1 + kz + ky * nk[2] + kx * nk[1] * nk[2]
 
 
+def q_path(nk):
+    """A q path through the k grid like a qdata_from_file list (kq_tools.f90:146-183): Gamma -> X along
+    x, X -> M along y, M -> R along z, then R -> Gamma along the diagonal; 1-based k indices."""
+    hx, hy, hz = nk[0] // 2, nk[1] // 2, nk[2] // 2
+    pts = [(i, 0, 0) for i in range(hx + 1)]
+    pts += [(hx, j, 0) for j in range(1, hy + 1)]
+    pts += [(hx, hy, k) for k in range(1, hz + 1)]
+    m = min(hx, hy, hz)
+    pts += [(hx * s // m, hy * s // m, hz * s // m) for s in range(m - 1, -1, -1)]
+    return np.array([k_index(a, b, c, nk) for a, b, c in pts], dtype=np.int32)
+
+
+def synthetic_vq(wl, q_data):
+    """V(q)(i1 = {i,k}, i2 = {l,j}) = vq(i,j,k,l) (read_vq, interaction_module.f90:106-119): a
+    density-density nearest-neighbour interaction V_ij(q) = V (cos qx + cos qy + cos qz)/3, full weight
+    within an orbital and 0.6 between orbitals, plus a small seeded exchange part vq(i,j,j,i) = 0.1 V_ij(q)
+    for i != j; real and symmetric under q -> -q."""
+    n = wl.n
+    q = np.asarray(q_data) - 1
+    nkx, nky, nkz = wl.nk
+    qz, qy, qx = q % nkz, (q // nkz) % nky, q // (nky * nkz)
+    c = (np.cos(2 * math.pi * qx / nkx) + np.cos(2 * math.pi * qy / nky) + np.cos(2 * math.pi * qz / nkz)) / 3.0
+    rng = np.random.default_rng([wl.seed, 9])
+    w = 0.6 + 0.05 * rng.normal(size=(n, n))
+    w = 0.5 * (w + w.T)
+    np.fill_diagonal(w, 1.0)
+    v = np.zeros((n * n, n * n, q.size), dtype=np.complex128)
+    for i in range(n):
+        for j in range(n):
+            v[i * n + i, j * n + j, :] = wl.vq * w[i, j] * c          # vq(i,j,i,j)
+            if i != j:
+                v[i * n + j, i * n + j, :] = 0.1 * wl.vq * w[i, j] * c  # vq(i,j,j,i): i1 = {i,j}, i2 = {i,j}
+    return v
+
+
 def generate_q_vol(nk, nq):
     """q list on a sub-grid of the k grid (kq_tools.f90:42-57), 1-based."""
     out = []
@@ -301,11 +358,15 @@ def build_inputs(wl: Workload) -> Inputs:
     # occupations (get_ndmft): n = 2 Re sum_{nu>=0} g / beta + 1/2
     n_dmft = (2.0 * np.real(giw[iwmax:].sum(axis=0)) / beta + 0.5).astype(np.complex128)
     u, ut = kanamori_u(n, wl.U, wl.J)
-    q_data = generate_q_vol(wl.nk, wl.qgrid)
-    k_eom = np.arange(1, wl.nkp + 1, dtype=np.int32)
+    q_data = q_path(wl.nk) if wl.q_list == "path" else generate_q_vol(wl.nk, wl.qgrid)
+    if wl.k_eom == "all":
+        k_eom = np.arange(1, wl.nkp + 1, dtype=np.int32)
+    else:
+        k_eom = np.arange(wl.nkp, 0, -3, dtype=np.int32)
+    v_q = np.asfortranarray(synthetic_vq(wl, q_data)) if wl.vq != 0.0 else None
     return Inputs(wl, np.asfortranarray(hk), np.asfortranarray(siw), np.asfortranarray(giw),
                   np.asfortranarray(dc), mu, beta, np.asfortranarray(u), np.asfortranarray(ut),
-                  n_dmft, q_data, k_eom)
+                  n_dmft, q_data, k_eom, v_q)
 
 
 def chi_loc(inp: Inputs, iwb: int, channel: str) -> np.ndarray:

@@ -18,6 +18,8 @@ shapes and index meaning (first index fastest, 0-based here):
 * chi contraction           susc_module.f90:34-56; main.F90:573-577, 661-668, 766-773
 * EOM (dynamic)             eom_module.f90:165-233
 * EOM (local, static)       eom_module.f90:39-161; main.F90:635-639
+* V(q) branch               main.F90:497-503 (read_vq per point), 710-719 (density BSE term);
+                            eom_module.f90:72-108 (Hartree/Fock), 183-207 (V terms of m_tot)
 * Sigma assembly            eom_module.f90:237-251; main.F90:840-855
 * chi assembly              main.F90:930-936, 1047-1063, 1124-1140
 * one-particle routines     one_particle_quant_module.f90:40-79 (get_giw), 124-180
@@ -25,9 +27,13 @@ shapes and index meaning (first index fastest, 0-based here):
 
 Parity pins (tests/test_oracle_*.py): golden vectors written by the reference's
 own Fortran modules (oracle/_ref, built from the reference sources with
-amdflang + MKL) on small workloads, and closed forms: the F = 0 limit, the
-local limit, a brute-force loop restatement on tiny grids, numpy inverse for the
-adjugate, and the Matsubara symmetries.
+amdflang + MKL) on small workloads (also with a reordered EOM k list, a q path
+and V(q)), and closed forms: the F = 0 limit, the local limit, a brute-force
+loop restatement on tiny grids, numpy inverse for the adjugate, the Matsubara
+symmetries; `finalize` (the chi and Sigma assembly after the loop, which the
+reference does inline in main.F90 and no golden vector covers) by the local
+limit (chi = beta^-2 sum_{nu nu'} chi_loc) and the F = 0 limit (chi = the full
+bubble, Sigma_DGA = siw).
 """
 from __future__ import annotations
 
@@ -280,6 +286,7 @@ class HotPath:
         chi = {r: self.calc_chi_qw(loc["gammaw_" + r], chi0nl) for r in ("d", "m")}  # main.F90:666-667
         big = {r: np.zeros((N, N), dtype=np.complex128) for r in ("d", "m")}
         gammaqd = np.zeros((n2, N), dtype=np.complex128)
+        v = self.v(iq)
         for d in range(nf):                                   # main.F90:679-720
             s = chi0q[:, :, d] * loc["winv"][:, d][None, :]
             s[np.arange(n2), np.arange(n2)] -= 1.0
@@ -289,6 +296,8 @@ class HotPath:
                 big[r][rows, :] = -rect
                 if r == "d":
                     gammaqd += rect
+            if v is not None:                                 # main.F90:710-719
+                big["d"][rows, :] -= (2.0 / self.beta ** 2 * (chi0q[:, :, d] @ v)) @ loc["oneplusgammaw_d"]
         res["gammaqd"] = gammaqd
         for r in ("d", "m"):
             a = big[r] + np.eye(N)                            # main.F90:726-729
@@ -300,19 +309,27 @@ class HotPath:
             res["chi_" + r] = chi[r]
         return res
 
+    def v(self, iq):
+        """V(q) of q list position iq (0-based) as read_vq returns it, or None without V(q)."""
+        return None if self.inp.v_q is None else self.inp.v_q[:, :, iq]
+
     # ---------------------------------------------------------------- A10
-    def mtot(self, etaqd, etaqm, gammaqd):
-        """m_tot of calc_eom_dynamic (eom_module.f90:187-201), no V(q)."""
+    def mtot(self, etaqd, etaqm, gammaqd, v=None, gammawd=None):
+        """m_tot of calc_eom_dynamic (eom_module.f90:183-207); with V(q) the density eta term takes
+        U + V - U~/2 and -V gamma^w_d / (beta nqp) is added."""
         inp = self.inp
         f = 1.0 / (self.beta * self.nqp)
-        uw = inp.u - 0.5 * inp.u_tilde
-        return (-f) * (uw @ etaqd) + 1.5 * f * (inp.u_tilde @ etaqm) + f * (inp.u @ gammaqd)
+        uw = inp.u - 0.5 * inp.u_tilde + (0.0 if v is None else v)
+        m = (-f) * (uw @ etaqd) + 1.5 * f * (inp.u_tilde @ etaqm) + f * (inp.u @ gammaqd)
+        if v is not None:
+            m = m - f * (v @ gammawd)
+        return m
 
-    def eom_dynamic(self, etaqd, etaqm, gammaqd, iq, iwb, sigma_nl):
+    def eom_dynamic(self, etaqd, etaqm, gammaqd, iq, iwb, sigma_nl, gammawd=None):
         """Sigma_nl(i,l,nu,k) += sum_{j,k'} m_tot({ji},{k'l,nu}) G(k-q,nu-omega)_{jk'}
         for nu >= 0, then Sigma(-nu-1) = Sigma(nu)^H (eom_module.f90:211-231)."""
         n, n2 = self.n, self.n2
-        m = self.mtot(etaqd, etaqm, gammaqd)
+        m = self.mtot(etaqd, etaqm, gammaqd, self.v(iq), gammawd)
         kidx = self.inp.k_data_eom - 1
         for d in range(self.iwf):                 # iwf = 0 .. iwfmax_small-1
             iwf = d
@@ -348,6 +365,28 @@ class HotPath:
                 acc = acc + (2.0 * u[i * n + i, j * n + j] - u[i * n + j, i * n + j]) * self.inp.n_dmft[j]
             sigma_dmft[i, i, :] += acc
 
+    @property
+    def n_fock(self):
+        """n_fock(nkp, n, n) of get_nfock (the Fock term's input, main.F90:271)."""
+        if getattr(self, "_nfock", None) is None:
+            self._nfock = get_nfock(self.inp)
+        return self._nfock
+
+    def eom_static_vq(self, iq, sigma_hf):
+        """The V(q) part of calc_eom_static (eom_module.f90:72-108): at q = Gamma (iq == 0) the Hartree
+        term 2 sum_j v({ii},{jj}) n_dmft(j) on the diagonal of every k; at every q the Fock term
+        sigma_hf(i,l,k) -= sum_{j,k'} v({ik'},{lj}) / nqp * n_fock(k-q, j, k')."""
+        n = self.n
+        v = self.v(iq)
+        if iq == 0:
+            for i in range(n):
+                for j in range(n):
+                    sigma_hf[i, i, :] += 2.0 * v[i * n + i, j * n + j] * self.inp.n_dmft[j]
+        kidx = self.inp.k_data_eom - 1
+        nf = self.n_fock[self.kq[kidx, iq]]                   # (K, j, k')
+        vv = v.reshape(n, n, n, n)                            # [i, k', l, j]
+        sigma_hf -= np.einsum("iklj,Kjk->ilK", vv, nf) / self.nqp
+
     # ---------------------------------------------------------------- the loop
     def run(self, points, chi_loc_fn, keep=()):
         """Run the loop body for `points` = [(iwb, iq0), ...] in the given order
@@ -365,6 +404,7 @@ class HotPath:
             "chi_loc_d": {}, "chi_loc_m": {}, "bubble_loc": {},
             "points": {},
         }
+        do_eom = self.inp.wl.do_eom
         cur = None
         loc = None
         for p, (iwb, iq) in enumerate(points):
@@ -375,15 +415,19 @@ class HotPath:
                     out["bubble_loc"][iwb] = loc["bubble_loc_tmp"]
                     out["chi_loc_d"][iwb] = loc["chi_loc_d"]
                     out["chi_loc_m"][iwb] = loc["chi_loc_m"]
-            if iwb == 0 and iq == 0:
+            if do_eom and iwb == 0 and iq == 0:                    # main.F90:635-637
                 self.eom_static_local(out["sigma_dmft"])
-            if iq == 0:
+            if do_eom and iwb == 0 and self.inp.v_q is not None:
+                self.eom_static_vq(iq, out["sigma_hf"])
+            if do_eom and iq == 0:                                 # main.F90:639
                 self.eom_dmft(loc["gammaw_d"], iwb, out["sigma_dmft"])
             res = self.point(iq, iwb, loc)
             out["chi_d"][:, :, p] = res["chi_d"]
             out["chi_m"][:, :, p] = res["chi_m"]
             out["bubble_nl"][:, :, p] = res["bubble_nl"]
-            self.eom_dynamic(res["eta_d"], res["eta_m"], res["gammaqd"], iq, iwb, out["sigma_nl"])
+            if do_eom:
+                self.eom_dynamic(res["eta_d"], res["eta_m"], res["gammaqd"], iq, iwb, out["sigma_nl"],
+                                 loc["gammaw_d"])
             if (iwb, iq) in keep:
                 out["points"][(iwb, iq)] = res
         return out

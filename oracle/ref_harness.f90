@@ -10,7 +10,11 @@
 ! (accumulate_chi0, get_chi0_loc(_inv), calc_chi_qw, inverse_matrix,
 ! calc_eom_dynamic, calc_eom_dmft, calc_eom_static) and zgemm where main.F90
 ! calls zgemm.  The input replaces read_vertex (hdf5_module.f90:629-773): the
-! local susceptibilities arrive already in its compound-index layout.
+! local susceptibilities arrive already in its compound-index layout, and V(q)
+! arrives per q point in the compound index read_vq (interaction_module.f90:
+! 39-123) produces.  Header flags: bit 0 do_vq, bit 1 k_path_eom (the EOM k list
+! is not the whole grid: kq_ind_eom from index_kq_eom, main.F90:366-369), bit 2
+! do_eom off (a q path, which the reference runs without the EOM, main.F90:341-347).
 !
 ! Usage: adga_ref_harness <input.bin> <output.bin>
 program adga_ref_harness
@@ -24,7 +28,7 @@ program adga_ref_harness
   character(len=512) :: fin, fout
   integer :: npts, flags, nw, p, iw, iwb, iq, iwf, d, a, b, c, er, cur
   integer :: qlo, nwu, slot
-  integer, allocatable :: pts_w(:), pts_q(:), kq_ind(:,:), wu(:)
+  integer, allocatable :: pts_w(:), pts_q(:), kq_ind(:,:), kq_ind_eom(:,:), wu(:)
   complex(kind=8), allocatable :: chil_d(:,:,:), chil_m(:,:,:)
   complex(kind=8), allocatable :: cwf_d(:,:), cwf_m(:,:), w_inv(:,:,:), chi0w(:,:,:), chi0q(:,:,:)
   complex(kind=8), allocatable :: gam_d(:,:), gam_m(:,:), opg_d(:,:), opg_m(:,:), gamq_d(:,:)
@@ -32,11 +36,11 @@ program adga_ref_harness
   complex(kind=8), allocatable :: sig_nl(:,:,:,:), sig_dmft(:,:,:), sig_hf(:,:,:), vq(:,:)
   complex(kind=8), allocatable :: x_d(:,:,:), x_m(:,:,:), bub_nl(:,:,:), x_loc_d(:,:,:), x_loc_m(:,:,:)
   complex(kind=8), allocatable :: bub_loc(:,:,:), first_gq(:,:), first_ed(:,:), first_em(:,:), kk(:,:)
-  complex(kind=8), allocatable :: chi0nl(:,:,:)
+  complex(kind=8), allocatable :: chi0nl(:,:,:), vqall(:,:,:)
   complex(kind=8) :: sw(64,64), loc_tmp(64,64), one, zero
   real(kind=8), parameter :: pi = 4d0*atan(1d0)
   character(len=200) :: erstr
-  logical :: nonlocal
+  logical :: nonlocal, kpath
 
   one = (1d0, 0d0)
   zero = (0d0, 0d0)
@@ -56,10 +60,12 @@ program adga_ref_harness
   dbgstr = ''
   verbstr = ''
   ounit = 0
-  do_vq = .false.
+  do_vq = iand(flags, 1) /= 0
+  kpath = iand(flags, 2) /= 0
   do_chi = .true.
-  do_eom = .true.
+  do_eom = iand(flags, 4) == 0
   nonlocal = .true.
+  text_output = .false.
   allocate(hk(ndim,ndim,nkp), siw(-iwmax:iwmax-1,ndim), giw(-iwmax:iwmax-1,ndim), dc(2,ndim))
   allocate(u(ndim2,ndim2), u_tilde(ndim2,ndim2), n_dmft(ndim), q_data(nqp), k_data_eom(nkp_eom))
   allocate(iw_data(-iwmax:iwmax-1))
@@ -73,6 +79,10 @@ program adga_ref_harness
   read(11) n_dmft
   read(11) q_data
   read(11) k_data_eom
+  if (do_vq) then
+     allocate(vqall(ndim2,ndim2,nqp))
+     read(11) vqall
+  endif
   read(11) pts_w
   read(11) pts_q
   ! local susceptibilities only for the omega values the point list uses
@@ -88,8 +98,17 @@ program adga_ref_harness
      iw_data(iw) = pi*(2*iw+1)/beta                    ! config_module.f90:510-512
   enddo
 
-  allocate(kq_ind(nkp,nqp))
-  call index_kq(kq_ind)                                 ! main.F90:364
+  allocate(kq_ind(nkp,nqp), kq_ind_eom(nkp_eom,nqp))
+  call index_kq(kq_ind)                                 ! main.F90:364-371
+  if (kpath) then
+     call index_kq_eom(kq_ind_eom)
+  else
+     kq_ind_eom = kq_ind
+  endif
+  if (do_vq) then                                       ! main.F90:262, 271
+     allocate(n_fock(nkp,ndim,ndim))
+     call get_nfock()
+  endif
   allocate(cwf_d(maxdim,maxdim), cwf_m(maxdim,maxdim), w_inv(ndim2,ndim2,-iwfmax_small:iwfmax_small-1))
   allocate(chi0w(ndim2,ndim2,-iwfmax_small:iwfmax_small-1), chi0q(ndim2,ndim2,-iwfmax_small:iwfmax_small-1))
   allocate(chi0nl(ndim2,ndim2,-iwfmax_small:iwfmax_small-1))
@@ -108,6 +127,11 @@ program adga_ref_harness
   do p = 1, npts
      iwb = pts_w(p)
      iq = pts_q(p)
+     if (do_vq) then                                      ! main.F90:497-503 (read_vq)
+        vq = vqall(:,:,iq)
+     else
+        vq = zero
+     endif
      if (iwb .ne. cur) then                               ! local update, main.F90:506-627
         cur = iwb
         do iwf = -iwfmax_small, iwfmax_small-1
@@ -155,8 +179,9 @@ program adga_ref_harness
         enddo
      endif
 
-     if (iwb .eq. 0 .and. iq .eq. 1) call calc_eom_static(kq_ind, iq, vq, sig_dmft, sig_hf, nonlocal)  ! main.F90:635-637
-     if (iq .eq. 1) call calc_eom_dmft(gam_d, gam_m, iwb, sig_dmft)                                      ! main.F90:639
+     if (do_eom .and. iwb .eq. 0 .and. (iq .eq. 1 .or. do_vq)) &                                           ! main.F90:635-637
+        call calc_eom_static(kq_ind_eom, iq, vq, sig_dmft, sig_hf, nonlocal)
+     if (do_eom .and. iq .eq. 1) call calc_eom_dmft(gam_d, gam_m, iwb, sig_dmft)                           ! main.F90:639
 
      chi0q = zero                                                     ! bubble, main.F90:644-657
      do iwf = iwstart, iwstop
@@ -191,6 +216,11 @@ program adga_ref_harness
         slab = cwf_m(qlo+1:qlo+ndim2,:)
         call zgemm('n','n',ndim2,maxdim,ndim2,one,sw,64,slab,ndim2,zero,rect,ndim2)
         bw_m(qlo+1:qlo+ndim2,:) = -rect
+        if (do_vq) then                                               ! main.F90:710-719
+           sw(1:ndim2,1:ndim2) = 2d0/(beta**2)*matmul(chi0q(:,:,iwf), vq)
+           call zgemm('n','n',ndim2,maxdim,ndim2,one,sw,64,opg_d,ndim2,zero,rect,ndim2)
+           bw_d(qlo+1:qlo+ndim2,:) = bw_d(qlo+1:qlo+ndim2,:) - rect
+        endif
      enddo
      do a = 1, maxdim                                                 ! inversion, main.F90:726-743
         bw_d(a,a) = bw_d(a,a) + 1d0
@@ -209,7 +239,7 @@ program adga_ref_harness
      deallocate(bw_d, bw_m)
      call calc_chi_qw(x_d(:,:,p), eta_d, chi0q)                       ! main.F90:766-769
      call calc_chi_qw(x_m(:,:,p), eta_m, chi0q)
-     call calc_eom_dynamic(eta_d, eta_m, gam_d, gamq_d, kq_ind, iwb, iq, vq, sig_nl)          ! main.F90:774-781
+     if (do_eom) call calc_eom_dynamic(eta_d, eta_m, gam_d, gamq_d, kq_ind_eom, iwb, iq, vq, sig_nl)  ! main.F90:774-781
      if (p .eq. 1) then
         first_gq = gamq_d
         first_ed = eta_d
@@ -229,5 +259,6 @@ program adga_ref_harness
   write(12) first_gq
   write(12) first_ed
   write(12) first_em
+  write(12) sig_hf
   close(12)
 end program adga_ref_harness

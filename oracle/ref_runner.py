@@ -36,17 +36,22 @@ def run(inp, points, chi_loc_fn, threads=1, timeout=3600, exe=None, args=()):
     npts = len(points)
     nkp_eom = len(inp.k_data_eom)
     wused = sorted({p[0] for p in points})
+    # header flags: bit 0 V(q), bit 1 EOM k list other than the grid in order, bit 2 no EOM (q path)
+    kpath = nkp_eom != int(np.prod(wl.nk)) or not np.array_equal(inp.k_data_eom, np.arange(1, nkp_eom + 1))
+    flags = (1 if inp.v_q is not None else 0) | (2 if kpath else 0) | (0 if wl.do_eom else 4)
     with tempfile.TemporaryDirectory() as td:
         fi = os.path.join(td, "in.bin")
         fo = os.path.join(td, "out.bin")
         with open(fi, "wb") as f:
             _w(f, [n, *wl.nk, inp.iwmax, wl.iwfmax_small, wl.iwbmax_small, len(inp.q_data),
-                   nkp_eom, npts, 0], np.int32)
+                   nkp_eom, npts, flags], np.int32)
             _w(f, [inp.beta, inp.mu], np.float64)
             for a in (inp.hk, inp.siw, inp.giw, inp.dc, inp.u, inp.u_tilde, inp.n_dmft):
                 _w(f, a, np.complex128)
             _w(f, inp.q_data, np.int32)
             _w(f, inp.k_data_eom, np.int32)
+            if inp.v_q is not None:
+                _w(f, inp.v_q, np.complex128)
             _w(f, [p[0] for p in points], np.int32)
             _w(f, [p[1] + 1 for p in points], np.int32)
             _w(f, [len(wused)], np.int32)
@@ -64,7 +69,7 @@ def run(inp, points, chi_loc_fn, threads=1, timeout=3600, exe=None, args=()):
     shapes = [("chi_d", (n2, n2, npts)), ("chi_m", (n2, n2, npts)), ("bubble_nl", (n2, n2, npts)),
               ("sigma_nl", (n, n, nf, nkp_eom)), ("sigma_dmft", (n, n, nf)),
               ("chi_loc_d", (n2, n2, nw)), ("chi_loc_m", (n2, n2, nw)), ("bubble_loc", (n2, n2, nw)),
-              ("gammaqd0", (n2, N)), ("eta_d0", (n2, N)), ("eta_m0", (n2, N))]
+              ("gammaqd0", (n2, N)), ("eta_d0", (n2, N)), ("eta_m0", (n2, N)), ("sigma_hf", (n, n, nkp_eom))]
     out = {}
     o = 0
     for name, shp in shapes:

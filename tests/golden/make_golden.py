@@ -32,6 +32,10 @@ FIXTURES = {
     "two_k4": ("two_k4", None),
     "t2g_k4": ("t2g_k4", None),
     "d5_k4": ("d5_k4", 24),
+    "t2g_k4_keom": ("t2g_k4_keom", None),    # EOM on a reordered third of the k grid, q on a 2^3 sub-grid
+    "t2g_k6_qpath": ("t2g_k6_qpath", None),  # chi along a q path (no EOM)
+    "two_k4_vq": ("two_k4_vq", None),        # V(q): density BSE term, Hartree/Fock, V terms of the EOM
+    "t2g_k4_vq": ("t2g_k4_vq", None),        # V(q) with the EOM on a reordered k list
 }
 
 
@@ -39,6 +43,13 @@ def input_digest(inp, points):
     h = hashlib.sha256()
     for a in (inp.hk, inp.siw, inp.giw, inp.u, inp.u_tilde, inp.n_dmft):
         h.update(np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).tobytes())
+    # the q and EOM k lists and V(q) enter only when they differ from the defaults, so the digests of the
+    # fixtures written before these inputs existed stay valid
+    if inp.wl.q_list != "vol" or inp.wl.k_eom != "all" or inp.v_q is not None:
+        h.update(np.ascontiguousarray(inp.q_data, dtype=np.int32).tobytes())
+        h.update(np.ascontiguousarray(inp.k_data_eom, dtype=np.int32).tobytes())
+        if inp.v_q is not None:
+            h.update(np.ascontiguousarray(inp.v_q).tobytes())
     for iwb in sorted({p[0] for p in points}):
         for ch in ("dens", "magn"):
             h.update(np.ascontiguousarray(synth.chi_loc(inp, iwb, ch)).tobytes())

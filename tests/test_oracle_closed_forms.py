@@ -193,3 +193,72 @@ def test_one_particle_constant_self_energy_shifts_the_band():
     assert rel_err(gl[:, 0, 0], adga_oracle.get_giw(shifted)[:, 0]) < 1e-13
     dcin = dataclasses.replace(base, dc=np.array([[s], [0.0]], dtype=np.complex128))
     assert rel_err(adga_oracle.get_nfock(dcin), adga_oracle.get_nfock(shifted)) < 1e-13
+
+
+def _local_limit_inputs():
+    wl = synth.workload("two_k4", nk=(1, 1, 1))
+    inp = synth.build_inputs(wl)
+    inp.hk[0, 1, 0] = inp.hk[1, 0, 0] = 0.0
+    hp = adga_oracle.HotPath(inp)
+    for iw in range(-inp.iwmax, inp.iwmax):
+        inp.giw[iw + inp.iwmax] = np.diag(hp.g_matrix(np.array([0]), iw)[0])
+    return wl, inp
+
+
+def test_finalize_local_limit_is_the_double_frequency_sum():
+    """One k point, orbital-diagonal H: chi_nl = 0 and bubble_nl = 0, so the assembled chi_r(q, omega)
+    (main.F90:935-936, 1050-1063, 1127-1140) is the local susceptibility, which is the double frequency
+    sum beta^-2 sum_{nu nu'} chi_loc,r({i nu}, {l nu'}) of the (transpose-symmetrised) generalized
+    susceptibility; Sigma_nl = 0, so Sigma_DGA is the DMFT self-energy on the diagonal.  A dropped
+    chi_loc or bubble_loc term, or chi_loc taken from the wrong omega, fails it."""
+    wl, inp = _local_limit_inputs()
+    hp = adga_oracle.HotPath(inp)
+    pts = [(w, 0) for w in range(-wl.iwbmax_small, wl.iwbmax_small + 1)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    out = hp.run(pts, cl)
+    fin = hp.finalize(out, pts)
+    n2, nf = wl.n ** 2, wl.nf
+    for p, (w, _) in enumerate(pts):
+        for r, ch in (("d", "dens"), ("m", "magn")):
+            x = cl(w, ch)
+            x = 0.5 * (x + x.T)
+            ref = np.asarray(x).reshape(nf, n2, nf, n2).sum(axis=(0, 2)) / inp.beta ** 2
+            assert rel_err(fin["chi_" + r][:, :, p], ref) < 1e-12, (w, r)
+    siw = inp.siw[inp.iwmax - wl.iwfmax_small:inp.iwmax + wl.iwfmax_small]
+    for a in range(wl.n):
+        assert rel_err(fin["sigma_dga"][a, a, :, 0], siw[:, a]) < 1e-14
+    assert abs(fin["sigma_dga"][0, 1]).max() < 1e-14
+
+
+def test_finalize_F_zero_is_the_full_bubble():
+    """chi_loc = chi0^w: chi_nl = 0 and chi_loc(omega) = 0, so the assembled chi_r(q, omega) is the
+    full bubble bubble_nl + bubble_loc = beta^-2 sum_nu chi0^q(nu) in both channels, and Sigma_DGA =
+    siw on the diagonal."""
+    inp = _inp("t2g_k4", noise=0.0)
+    wl = inp.wl
+    hp = adga_oracle.HotPath(inp)
+    pts = [(w, q) for w in (-1, 0, 2) for q in range(0, 64, 7)]
+    out = hp.run(pts, lambda w, c: synth.chi_loc(inp, w, c))
+    fin = hp.finalize(out, pts)
+    for p, (w, q) in enumerate(pts):
+        ref = hp.chi0q(q, w).sum(axis=2) / inp.beta ** 2
+        assert rel_err(fin["chi_d"][:, :, p], ref) < 1e-12
+        assert rel_err(fin["chi_m"][:, :, p], ref) < 1e-12
+    siw = inp.siw[inp.iwmax - wl.iwfmax_small:inp.iwmax + wl.iwfmax_small]
+    for a in range(wl.n):
+        assert np.abs(fin["sigma_dga"][a, a] - siw[:, a][:, None]).max() < 1e-14
+
+
+def test_vq_zero_equals_no_vq():
+    """V(q) = 0 given explicitly must reproduce the run without V(q) (every V term is linear in V),
+    and a q-independent V gives a k-independent Fock term."""
+    inp = _inp("t2g_k4_vq")
+    pts = [(0, 0), (0, 3), (1, 5)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    zero = dataclasses.replace(inp, v_q=np.zeros_like(inp.v_q))
+    none = dataclasses.replace(inp, v_q=None)
+    a = adga_oracle.HotPath(zero).run(pts, cl)
+    b = adga_oracle.HotPath(none).run(pts, cl)
+    for k in ("chi_d", "chi_m", "sigma_nl", "sigma_dmft"):
+        assert rel_err(a[k], b[k]) < 1e-14, k
+    assert np.abs(a["sigma_hf"]).max() == 0.0

@@ -9,7 +9,7 @@ from adga_amd import synth
 import adga_oracle
 from golden.make_golden import FIXTURES, input_digest
 
-CASES = ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8"]
+CASES = ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8", "t2g_k4_keom", "t2g_k6_qpath", "two_k4_vq", "t2g_k4_vq"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -23,6 +23,10 @@ def test_oracle_matches_reference(name):
     out = hp.run(pts, lambda w, c: synth.chi_loc(inp, w, c), keep={pts[0]})
     for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
         assert rel_err(out[k], gold[k]) < 1e-10, k
+    if "sigma_hf" in gold:
+        assert rel_err(out["sigma_hf"], gold["sigma_hf"]) < 1e-10
+    if not wl.do_eom:
+        assert np.abs(gold["sigma_nl"]).max() == 0.0 and np.abs(out["sigma_nl"]).max() == 0.0
     p0 = out["points"][pts[0]]
     assert rel_err(p0["gammaqd"], gold["gammaqd0"]) < 1e-10
     assert rel_err(p0["eta_d"], gold["eta_d0"]) < 1e-10
