@@ -44,7 +44,8 @@ def _dptr(a):
 
 
 class Context:
-    def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None, bubble=None, eom=None, lu_streams=None):
+    def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None, bubble=None, eom=None, lu_streams=None,
+                 do_eom=None, fallback=None):
         wl = inp.wl
         self.n = wl.n
         self.n2 = wl.n * wl.n
@@ -68,8 +69,11 @@ class Context:
             raise AdgaError(rc, "adga_ctx_create failed")
         self.h = h
         self._keep = None
+        if do_eom is None:
+            do_eom = getattr(wl, "do_eom", True)  # the reference runs a q path without the EOM
         for key, v in (("batch", batch), ("refine_tol", refine_tol), ("refine_maxit", refine_maxit),
-                       ("bubble", bubble), ("eom", eom), ("lu_streams", lu_streams)):
+                       ("bubble", bubble), ("eom", eom), ("lu_streams", lu_streams), ("do_eom", int(bool(do_eom))),
+                       ("fallback", fallback)):
             if v is not None:
                 self._check(lib.adga_set_option(self.h, key.encode(), float(v)))
 
@@ -176,11 +180,11 @@ class Context:
         self._check(lib.adga_reset_sigma(self.h))
 
     def stats(self):
-        out = np.zeros(14)
-        self._check(lib.adga_get_stats(self.h, _dptr(out), 14))
+        out = np.zeros(16)
+        self._check(lib.adga_get_stats(self.h, _dptr(out), 16))
         keys = ["refine_iters", "refine_resid", "t_local", "t_chi0", "t_gammas", "t_inversion", "t_eta_rest",
                 "t_eom", "t_chi", "t_lu_kernel", "n_lu_launches", "t_gemm_dominant", "flops_gemm_dominant",
-                "n_gemm_dominant"]
+                "n_gemm_dominant", "n_fp64_resolved", "n_failed_points"]
         return dict(zip(keys, out.tolist()))
 
 

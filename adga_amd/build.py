@@ -35,6 +35,7 @@ def _compile(src):
 
 
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+MKL_DIR = os.environ.get("ADGA_LAPACK_DIR", "/opt/conda/lib")
 FORTRAN = ["adga_c.f90", "lapack_module.f90", "adga_gpu.f90", "fortran_check.f90"]
 FCHECK = os.path.join(HERE, "fortran", "adga_fortran_check")
 
@@ -53,7 +54,9 @@ def build_fortran() -> str | None:
         if r.returncode != 0:
             raise RuntimeError(f"amdflang failed for {src}:\n{r.stderr[-4000:]}")
         objs.append(out)
-    r = subprocess.run([FC, "-o", FCHECK, *objs, "-L" + HERE, "-ladga_amd", "-Wl,-rpath,$ORIGIN/.."],
+    # LAPACK for lapack_module's host path above the GPU inverse's largest size (MKL of the image)
+    r = subprocess.run([FC, "-o", FCHECK, *objs, "-L" + HERE, "-ladga_amd", "-Wl,-rpath,$ORIGIN/..",
+                        "-L" + MKL_DIR, "-lmkl_rt", "-Wl,-rpath," + MKL_DIR],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran link failed:\n{r.stderr[-4000:]}")

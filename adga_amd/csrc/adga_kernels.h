@@ -42,9 +42,10 @@ template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
                int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
                hipEvent_t lag = nullptr, hipEvent_t join = nullptr);
+// `active` (may be null): per-matrix flags; matrices with a zero flag are skipped (Y left as is)
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
-                     const z64* R, z64* Y, bool accumulate, hipStream_t s);
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active = nullptr);
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
                 const z64* opgm, z64* Z, z64* B, hipStream_t s);
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
@@ -52,6 +53,10 @@ void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const
 void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
                     const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
                     z64* chi_out, z64* bub_out, hipStream_t s);
+// zeroes the per-point blocks (`len` complex values each) of the points whose flag is zero
+void launch_zero_points(int npts, long len, const int* keep, z64* a, hipStream_t s);
+// dst[i*len .. ) = src[sel[i]*len .. ) for i < k (gather) or the reverse (scatter)
+void launch_gather(int k, long len, const int* sel, const z64* src, z64* dst, bool scatter, hipStream_t s);
 void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
                       const z64* ndmft, z64* sdmft, hipStream_t s);
 void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s);

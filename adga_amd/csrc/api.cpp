@@ -18,15 +18,28 @@ using adga::c32;
 using adga::Geom;
 using adga::z64;
 
+constexpr int ADGA_NSTATS = 16;  // values adga_get_stats reports
+
+// Buffers of the FP64 re-solve of points whose complex64 refinement did not converge: the points'
+// S and right-hand sides are gathered into these, solved in complex128 and the solutions scattered back.
+struct FallbackBufs {
+  int cap = 0;  // points
+  z64 *S = nullptr, *B = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *M = nullptr, *inv = nullptr;
+  double* rel = nullptr;
+  int *piv = nullptr, *info = nullptr, *sel = nullptr;
+};
+
 struct adga_ctx {
   Geom g{};
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
   hipEvent_t ev_lag = nullptr, ev_join = nullptr;
   int lu_split = 2;                        // option "lu_streams": 2 splits each LU batch over two streams
+  hipEvent_t ev[7] = {}, ev_lu[2] = {};    // stage and LU timing of adga_run_points
   std::string err;
   int refine_maxit = 12;
   double refine_tol = 1e-12;
+  bool fallback = true;  // option "fallback": unconverged complex64 solves are solved again in complex128
   int batch = 256;
   int ldm = 0;
   // one-particle data (device)
@@ -48,6 +61,8 @@ struct adga_ctx {
   void* inv = nullptr;  // per-matrix panel inverses of the LU
   double* rel = nullptr;
   int *piv = nullptr, *info = nullptr, *qidx = nullptr;
+  int *act = nullptr, *keep = nullptr;  // per-matrix refinement flags, per-point "enters the self-energy" flags
+  FallbackBufs fb;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
   std::vector<int> ops_f[2];  // LU operation lists (host) per precision
   // accumulated self-energies
@@ -55,12 +70,13 @@ struct adga_ctx {
   // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
   int bubble_mode = 2;  // 0 direct, 1 DFT, 2 DFT when the batch has >= 64 points or the cache is valid
   int eom_mode = 1;     // 0 direct, 1 DFT (accumulated per omega, flushed on omega change / get_sigma)
+  bool do_eom = true;   // option "do_eom": the reference's do_eom (off for a q path, main.F90:341-347)
   z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
   bool bub_valid = false;
   int bub_iwb = 0;
   bool eom_pending = false;
   int eom_iwb = 0;
-  double stats[14] = {0};  // see adga_get_stats
+  double stats[ADGA_NSTATS] = {0};  // see adga_get_stats
   adga::LuTimer lut;       // events around the dominant GEMM kernel's launches
 };
 
@@ -114,8 +130,15 @@ void upload(z64* d, const double* h, size_t count, hipStream_t s) {
 void free_batch(adga_ctx* c) {
   dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
   dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->inv); dfree(c->rel);
-  dfree(c->piv); dfree(c->info); dfree(c->qidx);
+  dfree(c->piv); dfree(c->info); dfree(c->qidx); dfree(c->act); dfree(c->keep);
   c->cap = 0;
+}
+
+void free_fallback(adga_ctx* c) {
+  FallbackBufs& f = c->fb;
+  dfree(f.S); dfree(f.B); dfree(f.Y); dfree(f.Z); dfree(f.R); dfree(f.M); dfree(f.inv); dfree(f.rel);
+  dfree(f.piv); dfree(f.info); dfree(f.sel);
+  f.cap = 0;
 }
 
 void ensure_batch(adga_ctx* c, int nb) {
@@ -141,6 +164,8 @@ void ensure_batch(adga_ctx* c, int nb) {
   dalloc(&c->piv, 2 * nb * (size_t)g.N);
   dalloc(&c->info, 2 * nb);
   dalloc(&c->qidx, nb);
+  dalloc(&c->act, 2 * nb);
+  dalloc(&c->keep, nb);
   c->cap = nb;
 }
 
@@ -188,14 +213,128 @@ void ensure_ops(adga_ctx* c, int prec) {
   c->ops_f[prec] = adga::lu_ops_factor(N, N);
 }
 
+// FP64 re-solve of the points `pts` (batch positions) whose complex64 refinement did not converge:
+// their S and right-hand sides are gathered, the density and magnetic matrices rebuilt in complex128,
+// factored, solved and checked, and the solutions scattered back into the batch's Y.  Updates the
+// points' entries of relh / infoh and returns the number of matrices re-solved.
+int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>& relh, std::vector<int>& infoh) {
+  const Geom& g = c->g;
+  hipStream_t s = c->stream;
+  if (pts.empty()) return 0;
+  ensure_ops(c, 0);
+  FallbackBufs& f = c->fb;
+  const size_t per_mat = (size_t)g.N * c->ldm;
+  if (f.cap == 0) {
+    // at most 4 GB of complex128 matrices (two per point), at most 32 points per round
+    const size_t fit = ((size_t)4 << 30) / (2 * per_mat * sizeof(z64));
+    const int cap = (int)std::max<size_t>(1, std::min<size_t>(32, fit));
+    const size_t pmat = (size_t)g.N * g.n2;
+    dalloc(&f.S, (size_t)cap * g.nf * g.n4);
+    dalloc(&f.B, 2 * cap * pmat);
+    dalloc(&f.Y, 2 * cap * pmat);
+    dalloc(&f.Z, 2 * cap * pmat);
+    dalloc(&f.R, 2 * cap * pmat);
+    dalloc(&f.M, 2 * cap * per_mat);
+    dalloc(&f.inv, 2 * cap * adga::lu_inv_elems(g.N));
+    dalloc(&f.rel, 2 * cap);
+    dalloc(&f.piv, 2 * cap * (size_t)g.N);
+    dalloc(&f.info, 2 * cap);
+    dalloc(&f.sel, cap);
+    f.cap = cap;
+  }
+  const long slen = (long)g.nf * g.n4, blen = 2L * g.N * g.n2;
+  std::vector<double> r;
+  std::vector<int> inf;
+  int done = 0;
+  for (size_t i0 = 0; i0 < pts.size(); i0 += f.cap) {
+    const int k = (int)std::min<size_t>(f.cap, pts.size() - i0);
+    CK(hipMemcpyAsync(f.sel, &pts[i0], sizeof(int) * k, hipMemcpyHostToDevice, s));
+    adga::launch_gather(k, slen, f.sel, c->S, f.S, false, s);
+    adga::launch_gather(k, blen, f.sel, c->Bp, f.B, false, s);
+    adga::launch_build<double>(g, k, c->ldm, f.S, c->cd, c->cm, f.M, s);
+    if (!adga::launch_lu<double>(2 * k, g.N, g.N, c->ldm, c->ops_f[0], f.M, f.piv, f.inv, f.info, s) ||
+        !adga::launch_lu_solve<double>(2 * k, g.N, g.n2, c->ldm, f.M, f.piv, f.inv, f.B, f.Y, false, s))
+      throw HipError{hipErrorInvalidValue, "FP64 re-solve: BSE size outside the LU kernel's range"};
+    adga::launch_residual(g, k, f.S, c->cd, c->cm, f.B, f.Y, f.Z, f.R, f.rel, s);
+    adga::launch_gather(k, blen, f.sel, f.Y, c->Y, true, s);
+    r.resize(2 * k);
+    inf.resize(2 * k);
+    CK(hipMemcpyAsync(r.data(), f.rel, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(inf.data(), f.info, sizeof(int) * 2 * k, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j)
+      for (int ch = 0; ch < 2; ++ch) {
+        relh[2 * pts[i0 + j] + ch] = r[2 * j + ch];
+        infoh[2 * pts[i0 + j] + ch] = inf[2 * j + ch];
+      }
+    done += 2 * k;
+  }
+  return done;
+}
+
+// FP64 iterative refinement of the complex64 solves of a batch of nb points (2 nb matrices), per
+// matrix: a matrix stops when its relative residual max|r|/max|b| reaches refine_tol; it is handed to
+// the FP64 re-solve when its complex64 LU met an exactly zero pivot, a residual is not finite, a
+// correction did not at least halve its residual (stagnation or divergence: the complex64 factors
+// cannot precondition it, roughly cond(A) 2^-24 >= 1/2) or the iteration limit is reached.  The
+// corrections run only on the matrices still refining.  On return relh / infoh hold each matrix's
+// final residual and pivot status and resolved[m] = 1 for the matrices solved again in complex128
+// (both of a point; none when the option "fallback" is off); returns the number of corrections of
+// the slowest matrix.
+// infoh must have been queued for download (the complex64 LU's zero-pivot flags) on the stream.
+int refine(adga_ctx* c, int nb, std::vector<double>& relh, std::vector<int>& infoh, std::vector<int>& resolved) {
+  const Geom& g = c->g;
+  hipStream_t s = c->stream;
+  const int nmat = 2 * nb;
+  enum { ACTIVE = 0, DONE = 1, FP64 = 2 };
+  std::vector<int> state(nmat, ACTIVE), act(nmat);
+  std::vector<double> prev(nmat, INFINITY);
+  int iters = 0;
+  for (int it = 0;; ++it) {
+    adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
+    CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    int nact = 0;
+    for (int m = 0; m < nmat; ++m) {
+      if (state[m] != ACTIVE) continue;
+      const double r = relh[m];
+      if (infoh[m] != 0 || !std::isfinite(r)) state[m] = FP64;
+      else if (r <= c->refine_tol) state[m] = DONE;
+      else if (it >= c->refine_maxit || (it > 0 && r > 0.5 * prev[m])) state[m] = FP64;
+      else ++nact;
+      prev[m] = r;
+    }
+    if (nact == 0) break;
+    for (int m = 0; m < nmat; ++m) act[m] = state[m] == ACTIVE;
+    CK(hipMemcpyAsync(c->act, act.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, s));
+    // correction: solve with the complex64 factors for the (rounded) residual and add it to the FP64
+    // solution
+    adga::launch_lu_solve<float>(nmat, g.N, g.n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv, c->R, c->Y,
+                                 true, s, c->act);
+    iters = it + 1;
+  }
+  std::vector<int> pts;
+  resolved.assign(nmat, 0);
+  for (int b = 0; b < nb && c->fallback; ++b)
+    if (state[2 * b] == FP64 || state[2 * b + 1] == FP64) {
+      pts.push_back(b);
+      resolved[2 * b] = resolved[2 * b + 1] = 1;
+    }
+  c->stats[14] += fallback_fp64(c, pts, relh, infoh);
+  return iters;
+}
+
 }  // namespace
 
 namespace {
 // device buffer released on every exit path
+template <typename T = z64>
 struct DevBuf {
-  z64* p = nullptr;
+  T* p = nullptr;
   explicit DevBuf(size_t count) { dalloc(&p, count); }
   ~DevBuf() { dfree(p); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
 };
 int op_finish(adga_ctx* c, bool launched, const char* what) {
   if (!launched) {
@@ -251,6 +390,8 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     CK(hipStreamCreate(&c->stream2));
     CK(hipEventCreateWithFlags(&c->ev_lag, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (auto& e : c->ev) CK(hipEventCreate(&e));
+    for (auto& e : c->ev_lu) CK(hipEventCreate(&e));
     hipStream_t s = c->stream;
     dalloc(&c->hk, (size_t)g.n2 * g.nkp);
     dalloc(&c->siw, (size_t)2 * g.iwmax * g.n);
@@ -320,6 +461,18 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
     c->bubble_mode = (int)v;
     return ADGA_OK;
   }
+  if (!strcmp(key, "fallback")) {
+    if (!(v == 0 || v == 1)) return fail(c, ADGA_E_ARG, "fallback must be 0 or 1");
+    c->fallback = v == 1;
+    return ADGA_OK;
+  }
+  if (!strcmp(key, "do_eom")) {
+    if (!(v == 0 || v == 1)) return fail(c, ADGA_E_ARG, "do_eom must be 0 or 1");
+    if ((v == 1) != c->do_eom && c->eom_pending)
+      return fail(c, ADGA_E_STATE, "do_eom cannot change while an omega is pending (call adga_get_sigma)");
+    c->do_eom = v == 1;
+    return ADGA_OK;
+  }
   if (!strcmp(key, "eom")) {
     if (!(v == 0 || v == 1)) return fail(c, ADGA_E_ARG, "eom must be 0 (direct) or 1 (dft)");
     if ((int)v != c->eom_mode && c->eom_pending)
@@ -366,15 +519,12 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
     const Geom& g = c->g;
     hipStream_t s = c->stream;
     const int n2 = g.n2;
-    std::fill(c->stats, c->stats + 14, 0.0);
+    std::fill(c->stats, c->stats + ADGA_NSTATS, 0.0);
     c->lut.used = 0;
     c->lut.flops = 0.0;
     int worst = ADGA_OK;
-    hipEvent_t ev[8], lu0, lu1;
-    CK(hipEventCreate(&lu0));
-    CK(hipEventCreate(&lu1));
-    for (auto& e : ev) CK(hipEventCreate(&e));
-    std::vector<int> qh, infoh;
+    hipEvent_t* ev = c->ev;
+    std::vector<int> qh, infoh, st, keep, resolved;
     std::vector<double> relh;
     std::vector<z64> chih, bubh;
     for (int p0 = 0; p0 < npts; p0 += c->batch) {
@@ -384,10 +534,9 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       for (int b = 0; b < nb; ++b) qh[b] = c->q0[iq[p0 + b] - 1];
       CK(hipMemcpyAsync(c->qidx, qh.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
       const int nmat = 2 * nb;
-      const long tot = (long)nmat * g.N * n2;
       CK(hipEventRecord(ev[0], s));
       // local EOM for points at the Gamma point (main.F90:635-639)
-      for (int b = 0; b < nb; ++b)
+      for (int b = 0; b < nb && c->do_eom; ++b)
         if (iq[p0 + b] == 1)
           adga::launch_eom_local(g, c->iwb, c->iwb == 0, c->u, c->gamd, c->giw, c->ndmft, c->sig_dmft, s);
       CK(hipEventRecord(ev[1], s));
@@ -416,124 +565,125 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       double maxrel = 0.0;
       bool lu_ok;
       ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
+      relh.assign(nmat, 0.0);
+      infoh.assign(nmat, 0);
+      resolved.assign(nmat, 0);
       if (precision == ADGA_PREC_FP64) {
         adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
-        CK(hipEventRecord(lu0, s));
+        CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, c->ops_f[0], (z64*)c->M, c->piv,
                                         (z64*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
                                         c->ev_lag, c->ev_join);
-        CK(hipEventRecord(lu1, s));
+        CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->piv, (const z64*)c->inv,
                                                 c->Bp, c->Y, false, s);
         if (lu_ok) {
           // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
-          relh.resize(nmat);
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
           CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
+          CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
           CK(hipStreamSynchronize(s));
-          for (double r : relh)
-            if (std::isfinite(r)) maxrel = std::max(maxrel, r);
         }
       } else {
         adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
-        CK(hipEventRecord(lu0, s));
+        CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, c->ops_f[1], (c32*)c->M, c->piv,
                                        (c32*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
                                        c->ev_lag, c->ev_join);
-        CK(hipEventRecord(lu1, s));
+        CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv,
                                                c->Bp, c->Y, false, s);
-        relh.resize(nmat);
-        for (int it = 0; lu_ok; ++it) {
-          adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
-          CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
-          CK(hipStreamSynchronize(s));
-          maxrel = 0.0;
-          bool bad = false;
-          for (double r : relh) {
-            if (!std::isfinite(r)) bad = true;
-            else maxrel = std::max(maxrel, r);
-          }
-          if (bad || maxrel <= c->refine_tol || it >= c->refine_maxit) break;
-          // correction: solve with the complex64 factors for the (rounded) residual and add the
-          // correction to the FP64 solution
-          adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv, c->R,
-                                       c->Y, true, s);
-          iters = it + 1;
+        if (lu_ok) {
+          CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
+          iters = refine(c, nb, relh, infoh, resolved);
         }
       }
       if (!lu_ok) return fail(c, ADGA_E_ARG, "BSE size outside the supported range of the LU kernel");
       CK(hipGetLastError());
+      // per-point status; points that failed are left out of the self-energy
+      st.assign(nb, ADGA_OK);
+      keep.assign(nb, 1);
+      int nfailed = 0;
+      for (int b = 0; b < nb; ++b) {
+        for (int ch = 0; ch < 2; ++ch) {
+          const int m = 2 * b + ch;
+          int sm = ADGA_OK;
+          if (infoh[m] != 0) sm = ADGA_E_SINGULAR;
+          else if (!std::isfinite(relh[m])) sm = ADGA_E_NONFINITE;
+          else if (precision == ADGA_PREC_REFINED && !resolved[m] && relh[m] > c->refine_tol) sm = ADGA_E_NOCONV;
+          if (sm == ADGA_OK) maxrel = std::max(maxrel, relh[m]);
+          if (st[b] == ADGA_OK) st[b] = sm;
+        }
+        if (st[b] != ADGA_OK) {
+          keep[b] = 0;
+          ++nfailed;
+        }
+      }
       CK(hipEventRecord(ev[4], s));
       adga::launch_eta_chi(g, nb, c->Y, c->opgd, c->opgm, c->eta, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc,
                            c->chi_out, c->bub_out, s);
       CK(hipEventRecord(ev[5], s));
-      adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s);
-      if (c->eom_mode == 1) {
-        ensure_conv(c);
-        if (!c->macc) {
-          size_t nm = (size_t)g.iwf * g.n4 * g.nkp;
-          dalloc(&c->macc, nm);
-          dalloc(&c->mwork, nm);
-          CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, s));
+      if (c->do_eom) {
+        adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s);
+        if (nfailed > 0) {
+          CK(hipMemcpyAsync(c->keep, keep.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+          adga::launch_zero_points(nb, (long)n2 * g.N, c->keep, c->mt, s);
         }
-        if (c->eom_pending && c->eom_iwb != c->iwb) eom_flush(c);
-        adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s);
-        c->eom_pending = true;
-        c->eom_iwb = c->iwb;
-      } else {
-        adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, c->gt, c->sig_nl, s);
+        if (c->eom_mode == 1) {
+          ensure_conv(c);
+          if (!c->macc) {
+            size_t nm = (size_t)g.iwf * g.n4 * g.nkp;
+            dalloc(&c->macc, nm);
+            dalloc(&c->mwork, nm);
+            CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, s));
+          }
+          if (c->eom_pending && c->eom_iwb != c->iwb) eom_flush(c);
+          adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s);
+          c->eom_pending = true;
+          c->eom_iwb = c->iwb;
+        } else {
+          adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, c->gt, c->sig_nl, s);
+        }
       }
       CK(hipEventRecord(ev[6], s));
       CK(hipGetLastError());
       chih.resize((size_t)2 * nb * g.n4);
       bubh.resize((size_t)nb * g.n4);
-      infoh.resize(nmat);
       CK(hipMemcpyAsync(chih.data(), c->chi_out, sizeof(z64) * chih.size(), hipMemcpyDeviceToHost, s));
       CK(hipMemcpyAsync(bubh.data(), c->bub_out, sizeof(z64) * bubh.size(), hipMemcpyDeviceToHost, s));
-      CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
       CK(hipStreamSynchronize(s));
       float ms[6];
       for (int k = 0; k < 6; ++k) CK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
       c->stats[2 + 5] += ms[0];            // EOM (local part)
       c->stats[2 + 1] += ms[1];            // Chi0
       c->stats[2 + 2] += ms[2];            // Gammas (BSE build)
-      c->stats[2 + 3] += ms[3];            // Inversion (LU, solve, refinement)
+      c->stats[2 + 3] += ms[3];            // Inversion (LU, solve, refinement, FP64 re-solves)
       c->stats[2 + 6] += ms[4];            // Chi (eta + contraction)
       c->stats[2 + 5] += ms[5];            // EOM
       float lums = 0.f;
-      CK(hipEventElapsedTime(&lums, lu0, lu1));
+      CK(hipEventElapsedTime(&lums, c->ev_lu[0], c->ev_lu[1]));
       c->stats[9] += lums;                 // the batched LU kernel alone
       c->stats[10] += 1;                   // its launches
       c->stats[11] += c->lut.collect();    // the dominant GEMM kernel: milliseconds,
       c->stats[12] += c->lut.flops;        // algorithmic flops,
       c->stats[13] += c->lut.launches;     // launches
+      c->stats[15] += nfailed;             // points left out (non-zero status)
       c->lut.flops = 0.0;
       c->stats[0] = std::max(c->stats[0], (double)iters);
       c->stats[1] = std::max(c->stats[1], maxrel);
       for (int b = 0; b < nb; ++b) {
         const int p = p0 + b;
-        int st = ADGA_OK;
-        if (infoh[2 * b] != 0 || infoh[2 * b + 1] != 0) st = ADGA_E_SINGULAR;
-        if (st == ADGA_OK) {
-          double r = std::max(relh[2 * b], relh[2 * b + 1]);
-          if (!std::isfinite(r)) st = ADGA_E_NONFINITE;
-          else if (precision == ADGA_PREC_REFINED && r > c->refine_tol) st = ADGA_E_NOCONV;
-        }
-        if (status) status[p] = st;
-        if (st != ADGA_OK && worst == ADGA_OK) worst = st;
+        if (status) status[p] = st[b];
+        if (st[b] != ADGA_OK && worst == ADGA_OK) worst = st[b];
         if (chid) memcpy(chid + 2 * (size_t)p * g.n4, &chih[(size_t)(2 * b) * g.n4], sizeof(z64) * g.n4);
         if (chim) memcpy(chim + 2 * (size_t)p * g.n4, &chih[(size_t)(2 * b + 1) * g.n4], sizeof(z64) * g.n4);
         if (bubnl) memcpy(bubnl + 2 * (size_t)p * g.n4, &bubh[(size_t)b * g.n4], sizeof(z64) * g.n4);
       }
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(lu0);
-    (void)hipEventDestroy(lu1);
     if (worst == ADGA_E_SINGULAR) c->err = "singular BSE matrix (zero pivot in the LU)";
     else if (worst == ADGA_E_NOCONV) c->err = "iterative refinement did not reach the tolerance";
     else if (worst == ADGA_E_NONFINITE) c->err = "non-finite value in a BSE solve";
@@ -562,7 +712,7 @@ int adga_get_giw(adga_ctx* c, double* giw) {
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     const size_t ne = (size_t)2 * g.iwmax * g.n;
-    DevBuf d(ne);
+    DevBuf<> d(ne);
     const int rc = op_finish(c, adga::launch_op_giw(g, c->hk, c->siw, c->dc, d.p, c->stream), "get_giw");
     if (rc != ADGA_OK) return rc;
     // like get_giw, which overwrites the module's giw: the context's copy (the local bubble's input)
@@ -580,7 +730,7 @@ int adga_get_nfock(adga_ctx* c, double* n_fock) {
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     const size_t ne = (size_t)g.nkp * g.n2;
-    DevBuf d(ne);
+    DevBuf<> d(ne);
     const int rc = op_finish(c, adga::launch_op_nfock(g, c->hk, c->siw, c->dc, d.p, c->stream), "get_nfock");
     if (rc != ADGA_OK) return rc;
     CK(hipMemcpyAsync(n_fock, d.p, sizeof(z64) * ne, hipMemcpyDeviceToHost, c->stream));
@@ -598,7 +748,7 @@ int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc
   }
   return guarded(c, [&]() -> int {
     const size_t ns = (size_t)g.n2 * g.nf * g.nkp, nl = (size_t)g.n2 * g.nf, ng = (size_t)2 * g.iwmax * g.n2;
-    DevBuf ds(ns), dl(nl), dg(ng), dw(adga::op_sigma_loc_scratch(g));
+    DevBuf<> ds(ns), dl(nl), dg(ng), dw(adga::op_sigma_loc_scratch(g));
     upload(ds.p, sigma_sum, ns, c->stream);
     const int rc = op_finish(
         c, adga::launch_op_sigma_g_loc(g, c->hk, c->siw, c->dc, ds.p, dl.p, dg.p, dw.p, c->stream), "get_sigma_g_loc");
@@ -615,7 +765,7 @@ int adga_get_ndga(adga_ctx* c, const double* sigma_sum, double* n_dga_k, double*
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     const size_t ns = (size_t)g.n2 * g.nf * g.nkp_eom, nk = (size_t)g.nkp_eom * g.n2;
-    DevBuf ds(ns), dk(nk), dn((size_t)g.n);
+    DevBuf<> ds(ns), dk(nk), dn((size_t)g.n);
     upload(ds.p, sigma_sum, ns, c->stream);
     const int rc = op_finish(
         c, adga::launch_op_ndga(g, c->hk, c->siw, c->dc, c->keom, ds.p, dk.p, dn.p, c->stream), "get_ndga");
@@ -635,6 +785,7 @@ int adga_reset_sigma(adga_ctx* c) {
     CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, c->stream));
     if (c->macc) CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp, c->stream));
     c->eom_pending = false;
+    c->bub_valid = false;  // the next batch forms the bubble of its omega again
     CK(hipStreamSynchronize(c->stream));
     return ADGA_OK;
   });
@@ -642,7 +793,7 @@ int adga_reset_sigma(adga_ctx* c) {
 
 int adga_get_stats(adga_ctx* c, double* out, int nout) {
   if (!c || !out || nout < 0) return ADGA_E_ARG;
-  for (int i = 0; i < nout && i < 14; ++i) out[i] = c->stats[i];
+  for (int i = 0; i < nout && i < ADGA_NSTATS; ++i) out[i] = c->stats[i];
   return ADGA_OK;
 }
 
@@ -655,12 +806,17 @@ int adga_last_error(adga_ctx* c, char* buf, int len) {
 int adga_ctx_destroy(adga_ctx* c) {
   if (!c) return ADGA_OK;
   free_batch(c);
+  free_fallback(c);
   dfree(c->hk); dfree(c->siw); dfree(c->giw); dfree(c->dc); dfree(c->u); dfree(c->ut); dfree(c->ndmft);
   dfree(c->gt); dfree(c->keom);
   dfree(c->xd); dfree(c->xm); dfree(c->cd); dfree(c->cm); dfree(c->gamd); dfree(c->gamm); dfree(c->opgd);
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
   dfree(c->sig_nl); dfree(c->sig_dmft);
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
+  for (auto e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : c->ev_lu)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev_lag) (void)hipEventDestroy(c->ev_lag);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -672,6 +828,7 @@ int adga_ctx_destroy(adga_ctx* c) {
 int adga_inverse_matrix_z(int n, double* m, int* ierr) {
   if (n < 1 || !m || !ierr) return ADGA_E_ARG;
   *ierr = 0;
+  if (n > ADGA_INVERSE_MAX_N) return ADGA_E_ARG;  // the batched LU's largest size (pivot table in LDS)
   return guarded(nullptr, [&]() -> int {
     const int W = 2 * n, ldm = (W + 15) / 16 * 16;
     std::vector<z64> h((size_t)n * ldm);
@@ -682,26 +839,14 @@ int adga_inverse_matrix_z(int n, double* m, int* ierr) {
       for (int cc = 0; cc < n; ++cc) h[(size_t)r * ldm + cc] = src[(size_t)r * n + cc];
       for (int cc = n; cc < ldm; ++cc) h[(size_t)r * ldm + cc] = adga::mk(cc - n == r ? 1.0 : 0.0, 0.0);
     }
-    z64* dM = nullptr;
-    int *dpiv = nullptr, *dinfo = nullptr;
-    dalloc(&dM, h.size());
-    dalloc(&dpiv, (size_t)n);
-    dalloc(&dinfo, 1);
-    CK(hipMemcpy(dM, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
+    DevBuf<> dM(h.size()), dinv(adga::lu_inv_elems(n));
+    DevBuf<int> dpiv((size_t)n), dinfo(1);
+    CK(hipMemcpy(dM.p, h.data(), sizeof(z64) * h.size(), hipMemcpyHostToDevice));
     std::vector<int> ops = adga::lu_ops_factor(n, W);
-    z64* dinv = nullptr;
-    dalloc(&dinv, adga::lu_inv_elems(n));
-    bool ok = adga::launch_lu<double>(1, n, W, ldm, ops, dM, dpiv, dinv, dinfo, nullptr);
-    if (ok) {
-      CK(hipGetLastError());
-      CK(hipMemcpy(h.data(), dM, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
-      CK(hipMemcpy(ierr, dinfo, sizeof(int), hipMemcpyDeviceToHost));
-    }
-    dfree(dM);
-    dfree(dpiv);
-    dfree(dinfo);
-    dfree(dinv);
-    if (!ok) return ADGA_E_ARG;
+    if (!adga::launch_lu<double>(1, n, W, ldm, ops, dM.p, dpiv.p, dinv.p, dinfo.p, nullptr)) return ADGA_E_ARG;
+    CK(hipGetLastError());
+    CK(hipMemcpy(h.data(), dM.p, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ierr, dinfo.p, sizeof(int), hipMemcpyDeviceToHost));
     if (*ierr != 0) return ADGA_E_SINGULAR;
     z64* dst = reinterpret_cast<z64*>(m);
     for (int r = 0; r < n; ++r)

@@ -190,6 +190,35 @@ void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, cons
   hipLaunchKernelGGL(k_fill_neg, dim3((g.n2 * g.iwf + 63) / 64), dim3(64), 0, s, g, 1, sdmft);
 }
 
+__global__ void k_zero_points(int npts, long len, const int* __restrict__ keep, z64* __restrict__ a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)npts * len) return;
+  if (!keep[idx / len]) a[idx] = mk(0.0, 0.0);
+}
+
+void launch_zero_points(int npts, long len, const int* keep, z64* a, hipStream_t s) {
+  const long tot = (long)npts * len;
+  if (tot > 0)
+    hipLaunchKernelGGL(k_zero_points, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, npts, len, keep, a);
+}
+
+__global__ void k_gather(int k, long len, const int* __restrict__ sel, const z64* __restrict__ src,
+                         z64* __restrict__ dst, int scatter) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)k * len) return;
+  const long i = idx / len, e = idx % len;
+  const long far = (long)sel[i] * len + e;
+  if (scatter) dst[far] = src[idx];
+  else dst[idx] = src[far];
+}
+
+void launch_gather(int k, long len, const int* sel, const z64* src, z64* dst, bool scatter, hipStream_t s) {
+  const long tot = (long)k * len;
+  if (tot > 0)
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, k, len, sel, src, dst,
+                       scatter ? 1 : 0);
+}
+
 void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s) {
   long tot = (long)g.n2 * g.iwf * nk;
   hipLaunchKernelGGL(k_fill_neg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nk, sigma);

@@ -579,13 +579,15 @@ constexpr int SOLVE_LDS_BYTES_WIDE = 163840;  // one workgroup per CU (the whole
 template <typename T, int CG, int LDSB>
 __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ldm, const cplx<T>* __restrict__ Mall,
                                                 const int* __restrict__ permall, const cplx<T>* __restrict__ invall,
-                                                const z64* __restrict__ R, z64* __restrict__ Y, int accumulate) {
+                                                const z64* __restrict__ R, z64* __restrict__ Y, int accumulate,
+                                                const int* __restrict__ active) {
   using C = cplx<T>;
   // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
   // columns without predicates
   __shared__ C x[LDSB / sizeof(C)];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
   const int m = blockIdx.x, s0 = blockIdx.y * ns, nsg = min(ns, nrhs - s0);
+  if (active && !active[m]) return;  // the whole workgroup leaves: no barrier is reached
   const C* M = Mall + (long)m * N * ldm;
   const int* perm = permall + (long)m * N;
   const int nb = (N + P - 1) / P;
@@ -1039,7 +1041,7 @@ LuTimer::~LuTimer() {
 // at two workgroups per CU.  False when not even one column fits.
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
-                     const z64* R, z64* Y, bool accumulate, hipStream_t s) {
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active) {
   static const int cgs[] = {1, 2, 4, 9, 16};
   auto pick = [&](int ldsb, bool whole) {
     const int cap = ldsb / (int)sizeof(cplx<T>) / (N + P);  // columns that fit, with P zero rows
@@ -1064,7 +1066,7 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
 #define ADGA_SOLVE_CASE(CGV, LDSB)                                                                              \
   if (cg == CGV) {                                                                                              \
     hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
-                       accumulate ? 1 : 0);                                                                     \
+                       accumulate ? 1 : 0, active);                                                             \
     return true;                                                                                                \
   }
   if (wide) {
@@ -1082,8 +1084,8 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
   return false;
 }
 template bool launch_lu_solve<float>(int, int, int, int, const c32*, const int*, const c32*, const z64*, z64*, bool,
-                                     hipStream_t);
+                                     hipStream_t, const int*);
 template bool launch_lu_solve<double>(int, int, int, int, const z64*, const int*, const z64*, const z64*, z64*, bool,
-                                      hipStream_t);
+                                      hipStream_t, const int*);
 
 }  // namespace adga

@@ -95,6 +95,7 @@ program adga_fortran_check
   write(12) zeros
   write(12) zeros
   write(12) zeros
+  write(12) sig_hf
   close(12)
 
   ! drop-in lapack_module: A A^-1 = 1 for a diagonally dominant complex matrix, and the

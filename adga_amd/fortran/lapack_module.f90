@@ -2,7 +2,9 @@
 ! same module name, same public generic `inverse_matrix` with the same specific
 ! signatures, so main.F90 and the other modules compile against it unchanged.
 ! The complex inverse runs on the GPU (LU with partial pivoting and n solves,
-! libadga_amd.so); errors keep the reference's contract: ierr /= 0 and a
+! libadga_amd.so) up to n = 4096 (the batched LU's largest size); a larger matrix
+! is inverted on the host with LAPACK's zgetrf + zgetri, which the reference's own
+! build links anyway.  Errors keep the reference's contract: ierr /= 0 and a
 ! message in erstr ("ERROR in ZGETRF" for an exactly singular matrix).
 module lapack_module
   use, intrinsic :: iso_c_binding
@@ -10,6 +12,8 @@ module lapack_module
   implicit none
   private
   public inverse_matrix
+
+  integer, parameter :: gpu_max_n = 4096   ! ADGA_INVERSE_MAX_N of include/adga_amd.h
 
   interface inverse_matrix
     module procedure inverse_matrix_d, inverse_matrix_z
@@ -32,6 +36,10 @@ contains
       erstr = "ERROR: inverse_matrix needs a square matrix"
       return
     endif
+    if (n > gpu_max_n) then
+      call inverse_host(M, erstr, ierr)
+      return
+    endif
     allocate(work(n, n))
     work = M                                 ! contiguous copy (M may be a strided section)
     rc = c_adga_inverse_matrix_z(int(n, c_int), work, info)
@@ -47,6 +55,37 @@ contains
     endif
     deallocate(work)
   end subroutine inverse_matrix_z
+
+  ! zgetrf + workspace query + zgetri on the host, for sizes above the GPU path's limit
+  subroutine inverse_host(M, erstr, ierr)
+    double complex, intent(inout)   :: M(:,:)
+    integer, intent(out)            :: ierr
+    character(len=200), intent(out) :: erstr
+    double complex, allocatable :: a(:,:), work(:)
+    double complex :: wq(1)
+    integer, allocatable :: ipiv(:)
+    integer :: n, lwork
+
+    n = size(M, 1)
+    allocate(a(n, n), ipiv(n))
+    a = M
+    call zgetrf(n, n, a, n, ipiv, ierr)
+    if (ierr /= 0) then
+      erstr = "ERROR in ZGETRF"
+      deallocate(a, ipiv)
+      return
+    endif
+    call zgetri(n, a, n, ipiv, wq, -1, ierr)
+    lwork = max(1, int(dble(wq(1))))
+    allocate(work(lwork))
+    call zgetri(n, a, n, ipiv, work, lwork, ierr)
+    if (ierr /= 0) then
+      erstr = "ERROR in ZGETRI"
+    else
+      M = a
+    endif
+    deallocate(a, ipiv, work)
+  end subroutine inverse_host
 
   subroutine inverse_matrix_d(M, erstr, ierr)
     double precision, intent(inout) :: M(:,:)

@@ -67,11 +67,12 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
 
     for s in range(warmup):
         step(s)
-    ctx.reset_sigma()
+    ctx.reset_sigma()  # also drops the bubble cache: every omega's bubble is formed inside the timed region
     cur[0] = None  # run_points returns with its stream drained, so the device is idle here
     lu_ms, lu_n, iters, resid = 0.0, 0, 0.0, 0.0
     lu_mats = 0
     g_ms, g_flops, g_n = 0.0, 0.0, 0
+    resolved, failed = 0, 0
     outs = []
     npts = 0
     t0 = time.perf_counter()
@@ -85,6 +86,8 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
         g_flops += st["flops_gemm_dominant"]
         g_n += int(st["n_gemm_dominant"])
         iters = max(iters, st["refine_iters"])
+        resolved += int(st["n_fp64_resolved"])
+        failed += int(st["n_failed_points"])
         resid = max(resid, st["refine_resid"])
         npts += r["chi_d"].shape[2]
         outs.append(r)
@@ -92,23 +95,37 @@ def run_mode(api, synth, inp, precision, batch, steps, warmup, omegas, chi_cache
     t1 = time.perf_counter()
     ctx.close()
     return {"seconds": t1 - t0, "points": npts, "lu_ms": lu_ms, "lu_launches": lu_n, "lu_mats": lu_mats, "iters": iters,
-            "gemm_ms": g_ms, "gemm_flops": g_flops, "gemm_launches": g_n,
+            "gemm_ms": g_ms, "gemm_flops": g_flops, "gemm_launches": g_n, "resolved": resolved, "failed": failed,
             "resid": resid, "chi_d": np.concatenate([o["chi_d"] for o in outs], axis=2),
             "chi_m": np.concatenate([o["chi_m"] for o in outs], axis=2), "sigma_nl": sigma}
 
 
-def cpu_baseline(synth, inp, chi_cache, npoints):
-    """The reference's own Fortran modules (oracle/_ref) on one host core, or the numpy oracle when the
-    reference binary is absent; per-point time from the difference of a 1-point and a (1+npoints)-point run
-    (removes the harness's set-up)."""
+def cpu_baseline(synth, inp, chi_cache, npoints, procs):
+    """The reference's own Fortran modules (oracle/_ref) on `procs` host cores, one single-threaded process
+    per core on its own block of q points (the reference's rank-per-core model, src/mpi_org.F90:65-98), or
+    the numpy oracle on one core when the reference binary is absent.  Each process's per-point time is the
+    difference of a 1-point and a (1+npoints)-point run (removes the harness's set-up); the throughput is
+    the sum over the processes."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ref_runner
+    from concurrent.futures import ThreadPoolExecutor
     cl = lambda w, c: chi_cache(w)[0 if c == "dens" else 1]  # noqa: E731
     if ref_runner.available():
-        t1 = ref_runner.run(inp, [(0, 0)], cl, threads=1)["wall_s"]
-        tn = ref_runner.run(inp, [(0, q) for q in range(npoints + 1)], cl, threads=1)["wall_s"]
-        kind = "reference"
-        desc = "oracle/_ref (reference src/*_module.f90 via amdflang + MKL, 1 thread)"
+        nq = len(inp.q_data)
+
+        def block(i, m):
+            return [(0, (i * (npoints + 1) + j) % nq) for j in range(m)]
+
+        with ThreadPoolExecutor(procs) as ex:
+            t1 = list(ex.map(lambda i: ref_runner.run(inp, block(i, 1), cl, threads=1)["wall_s"], range(procs)))
+            tn = list(ex.map(lambda i: ref_runner.run(inp, block(i, npoints + 1), cl, threads=1)["wall_s"],
+                             range(procs)))
+        rate = sum(npoints / max(b - a, 1e-9) for a, b in zip(t1, tn))
+        per = procs / rate
+        return {"value": rate, "unit": "points/s", "cores": procs, "kind": "reference",
+                "sample": f"{procs} single-threaded processes of oracle/_ref (reference src/*_module.f90 via "
+                          f"amdflang + MKL), each on {npoints} (q, omega) points at omega=0 of the same workload "
+                          f"({procs * npoints} points); {per:.3f} s per point per process"}
     else:
         import adga_oracle
         from threadpoolctl import threadpool_limits
@@ -120,11 +137,9 @@ def cpu_baseline(synth, inp, chi_cache, npoints):
             a = time.perf_counter()
             hp.run([(0, q) for q in range(npoints + 1)], cl)
             tn = time.perf_counter() - a
-        kind = "port"
-        desc = "numpy oracle, 1 thread"
     per = max((tn - t1) / npoints, 1e-9)
-    return {"value": 1.0 / per, "unit": "points/s", "cores": 1, "kind": kind,
-            "sample": f"{npoints} (q,omega) points at omega=0 of the same workload, {desc}; "
+    return {"value": 1.0 / per, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"{npoints} (q,omega) points at omega=0 of the same workload, numpy oracle, 1 thread; "
                       f"{per:.3f} s/point"}
 
 
@@ -173,6 +188,22 @@ def one_particle(api, synth, iwmax=2000, reps=3, cpu_iwmax=200, cpu=True):
     return out
 
 
+def level1_inverse(api, N, reps=3):
+    """The one-matrix drop-in for inverse_matrix_z (adga_inverse_matrix_z; INTEGRATION.md level 1): best-of-
+    `reps` wall time of one N x N complex128 inverse through the C ABI, host copies included, and the
+    (q, omega) rate it would cap the loop at (two inverses per point, everything else free)."""
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)) + 4 * np.sqrt(N) * np.eye(N)
+    api.inverse_matrix(a)
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        api.inverse_matrix(a)
+        best = min(best, time.perf_counter() - t0)
+    return {"N": N, "ms_per_inverse": round(best * 1e3, 3), "points_per_s_cap": round(1.0 / (2 * best), 2),
+            "note": "one matrix per call (one workgroup per panel), hipMalloc/hipFree and a host transpose per call"}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes` (reads and writes `nbytes` each), outside any timed region."""
     import torch
@@ -194,6 +225,8 @@ def main():
     ap.add_argument("--batch", type=int, default=2048)
     ap.add_argument("--workload", default="t2g_k20")
     ap.add_argument("--cpu-points", type=int, default=6)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
+                    help="single-threaded reference processes of the CPU baseline (one per core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lu-streams", type=int, default=None, help="the library's lu_streams option (1 or 2)")
@@ -276,6 +309,8 @@ def main():
                              "matrices_timed": ref["lu_mats"], "factorisations_timed": ref["lu_launches"]},
         "refine_iters_max": ref["iters"],
         "refine_resid_max": ref["resid"],
+        "fp64_resolved_matrices": ref["resolved"],
+        "failed_points": ref["failed"],
     }
     if fp is not None:
         fval = fp["points"] / fp["seconds"]
@@ -290,11 +325,17 @@ def main():
                               "lu_frac_of_fp64_peak": round(frate / PEAK["fp64"], 4)}
         line["refined_over_fp64"] = round(value / fval, 3)
         line["rel_err_refined_vs_fp64"] = errs
+        line["parity_ok"] = bool(max(errs.values()) <= 1e-10 and ref["failed"] == 0)
+    line["level1_inverse"] = level1_inverse(api, N)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(synth, inp, chi_cache, args.cpu_points)
+        line["cpu_baseline"] = cpu_baseline(synth, inp, chi_cache, args.cpu_points, args.cpu_procs)
     if not args.no_one_particle:
         line["one_particle"] = one_particle(api, synth, cpu=not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
+    if line.get("parity_ok") is False:
+        print("refined results differ from the plain FP64 path by more than 1e-10 (or points failed)",
+              file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -26,11 +26,15 @@ enum {
   ADGA_E_ARG = 1,        /* invalid argument or size */
   ADGA_E_HIP = 2,        /* HIP runtime failure (no device, launch failure, ...) */
   ADGA_E_SINGULAR = 3,   /* exactly zero pivot in a BSE LU: the reference's "ERROR in ZGETRF" */
-  ADGA_E_NOCONV = 4,     /* refined mode: residual above tolerance after the iteration limit */
+  ADGA_E_NOCONV = 4,     /* refined mode with the option "fallback" off: residual above tolerance */
   ADGA_E_NONFINITE = 5,  /* NaN/Inf met in a BSE matrix or residual */
   ADGA_E_STATE = 6,      /* call out of order (e.g. run_points before set_local_omega) */
   ADGA_E_NOMEM = 7       /* device allocation failed */
 };
+
+/* largest n adga_inverse_matrix_z takes (larger: ADGA_E_ARG; the Fortran lapack_module falls back to the
+   host LAPACK there) */
+#define ADGA_INVERSE_MAX_N 4096
 
 enum {
   ADGA_PREC_FP64 = 0,    /* complex128 LU and solve */
@@ -66,7 +70,11 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
 
 /*
  * Options: "batch" (points per device batch, default 256), "refine_tol" (refined mode: relative
- * residual max|r|/max|b| every BSE solve must reach, default 1e-12), "refine_maxit" (default 12),
+ * residual max|r|/max|b| every BSE solve must reach, default 1e-12), "refine_maxit" (default 12;
+ * a matrix that has not converged by then, or whose residual a correction did not halve, or whose
+ * complex64 LU met a zero pivot, is solved again in complex128 unless "fallback" is 0; its final
+ * status then depends only on the complex128 solve), "do_eom" (1: the default; 0: no
+ * equation of motion, as the reference's do_eom = .false. for a q path),
  * "bubble" (0: direct k sum per point; 1: all q of the current omega at once by
  * k-grid DFT, cached per omega; 2: DFT when a batch has >= 64 points or the cache is valid, the
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
@@ -97,7 +105,9 @@ int adga_set_local_omega(adga_ctx *ctx, int iwb, const double *chi_loc_dens, con
  *   iq(npts): 1-based positions in q_data.
  *   chi_dens_out/chi_magn_out/bubble_nl_out(ndim^2,ndim^2,npts): the non-local chi parts and the
  *   non-local bubble, as chi_qw_dens/magn(:,:,iqw) and bubble_nl(:,:,iqw) of the reference.
- *   status(npts): per-point ADGA_* code (may be NULL).
+ *   status(npts): per-point ADGA_* code (may be NULL).  A point with a non-zero status (singular or
+ *   non-finite BSE matrix, or no convergence) is left out of the self-energy; its chi outputs are
+ *   whatever the failed solve gave.  The other points of the batch are not affected.
  * Replaces: the loop body src/main.F90:635-781 (accumulate_chi0, the inline BSE zgemm calls,
  * inverse_matrix, calc_chi_qw, calc_eom_dynamic/_dmft/_static) for a batch of points.
  */
@@ -112,7 +122,8 @@ int adga_run_points(adga_ctx *ctx, int npts, const int *iq, int precision, doubl
  */
 int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *sigma_hf);
 
-/* Zeroes the accumulated self-energies. */
+/* Zeroes the accumulated self-energies and drops the per-omega bubble cache (the next batch forms the
+   bubble of its omega again). */
 int adga_reset_sigma(adga_ctx *ctx);
 
 /*
@@ -155,12 +166,13 @@ int adga_get_sigma_g_loc(adga_ctx *ctx, const double *sigma_sum, double *sigma_l
 int adga_get_ndga(adga_ctx *ctx, const double *sigma_sum, double *n_dga_k, double *n_dga);
 
 /*
- * Statistics of the last adga_run_points call (nout <= 14 values): out[0] = max refinement
- * iterations, out[1] = max final relative residual, out[2..8] = milliseconds spent in the
- * reference's stage buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM, Chi
- * (src/main.F90:484-783), out[9] = milliseconds of the batched LU factorisations, out[10] = their
- * number, out[11..13] = milliseconds, algorithmic flops and launches of the factorisation's
- * dominant GEMM kernel (the largest tile shape).
+ * Statistics of the last adga_run_points call (nout <= 16 values): out[0] = max refinement
+ * iterations, out[1] = max final relative residual of the points that succeeded, out[2..8] =
+ * milliseconds spent in the reference's stage buckets Local, Chi0, Gammas, Inversion, Eta rest, EOM,
+ * Chi (src/main.F90:484-783), out[9] = milliseconds of the batched LU factorisations, out[10] =
+ * their number, out[11..13] = milliseconds, algorithmic flops and launches of the factorisation's
+ * dominant GEMM kernel (the largest tile shape), out[14] = matrices solved again in complex128
+ * after their complex64 refinement failed, out[15] = points with a non-zero status.
  */
 int adga_get_stats(adga_ctx *ctx, double *out, int nout);
 
@@ -171,7 +183,8 @@ int adga_ctx_destroy(adga_ctx *ctx);
 
 /*
  * In-place inverse of a complex(kind=8) n x n column-major matrix on the GPU (LU with partial
- * pivoting, then n solves); ierr > 0 is the first zero pivot like zgetrf's info.
+ * pivoting, then n solves); ierr > 0 is the first zero pivot like zgetrf's info.  n above
+ * ADGA_INVERSE_MAX_N returns ADGA_E_ARG.
  * Replaces: inverse_matrix_z (src/lapack_module.f90:40-70).
  */
 int adga_inverse_matrix_z(int n, double *m, int *ierr);

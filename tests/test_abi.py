@@ -41,4 +41,5 @@ def test_argument_validation_without_gpu():
     assert lib.adga_ctx_create(C.byref(d), i, i, z, z, z, z, z, z, z, C.byref(h)) == _lib.ADGA_E_ARG
     ierr = C.c_int(0)
     assert lib.adga_inverse_matrix_z(0, z, C.byref(ierr)) == _lib.ADGA_E_ARG
+    assert lib.adga_inverse_matrix_z(4097, z, C.byref(ierr)) == _lib.ADGA_E_ARG  # above ADGA_INVERSE_MAX_N
     assert lib.adga_ctx_destroy(None) == _lib.ADGA_OK

@@ -26,6 +26,9 @@ def _api():
 
 def _compare_golden(out, gold, wl, tol):
     for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+        if not wl.do_eom and k.startswith("sigma"):
+            assert np.abs(out[k]).max() == 0.0, k  # no EOM on a q path, as in the reference
+            continue
         e = rel_err(out[k], gold[k])
         assert e < tol, (k, e)
     for w in out["chi_loc_d"]:
@@ -40,7 +43,7 @@ MODES = [(1, 1), (0, 0), (1, 0), (0, 1)]  # (bubble, eom): 1 = k-grid DFT, 0 = d
 
 @pytest.mark.parametrize("modes", MODES, ids=lambda m: f"bubble{m[0]}-eom{m[1]}")
 @pytest.mark.parametrize("precision", ["fp64", "refined"])
-@pytest.mark.parametrize("name", ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8"])
+@pytest.mark.parametrize("name", ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8", "t2g_k4_keom", "t2g_k6_qpath"])
 def test_gpu_matches_reference_golden(name, precision, modes):
     from golden.make_golden import FIXTURES
     api = _api()
@@ -251,3 +254,54 @@ def test_gpu_five_orbital_large_box_against_oracle():
         for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
             e = rel_err(out[k], ref[k])
             assert e < 1e-10, (precision, k, e)
+
+
+def test_gpu_refinement_hands_unconverged_matrices_to_fp64():
+    """refine_maxit = 1 with a tolerance one complex64 correction cannot reach: every matrix is solved
+    again in complex128, so the refined mode returns the plain FP64 path's results (the same kernels on
+    the same S and C) and counts two re-solved matrices per point."""
+    api = _api()
+    inp = synth.build_inputs(synth.workload("t2g_k4"))
+    pts = [(1, q) for q in range(0, 64, 5)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    with api.Context(inp, refine_maxit=1, refine_tol=1e-17) as ctx:
+        out = api.run_loop(ctx, pts, cl, "refined")
+        st = ctx.stats()
+    with api.Context(inp) as ctx:
+        ref = api.run_loop(ctx, pts, cl, "fp64")
+    assert st["n_fp64_resolved"] == 2 * len(pts)
+    assert st["n_failed_points"] == 0 and (out["status"] == 0).all()
+    for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
+        assert rel_err(out[k], ref[k]) < 1e-13, k
+
+
+def test_gpu_near_instability_falls_back_to_fp64():
+    """A vertex 1e5 times the default noise at N = 1620 puts A near singular (condition 4e8 for the
+    density matrix of q = Gamma, 1e7 elsewhere): complex64 factors cannot precondition it (cond * 2^-24
+    > 1), the refinement stagnates or diverges and the matrix is solved again in complex128.  Every
+    point succeeds; the refined mode agrees with the plain FP64 path, and both with the oracle's explicit
+    LAPACK inverse within the conditioning bound cond * eps ~ 1e-7 that any two FP64 formulations share
+    (1e-10 is out of reach for any method at this condition number)."""
+    import adga_oracle
+    api = _api()
+    wl = synth.workload("t2g_k4", noise=100.0, iwfmax_small=90)
+    inp = synth.build_inputs(wl)
+    pts = [(1, 0), (1, 3), (1, 17)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    print("oracle N = 1620, noise 100", flush=True)
+    ref = adga_oracle.HotPath(inp).run(pts, cl)
+    res = {}
+    for precision in ("fp64", "refined"):
+        print(f"near instability {precision}", flush=True)
+        with api.Context(inp, refine_tol=1e-14) as ctx:
+            res[precision] = api.run_loop(ctx, pts, cl, precision)
+            st = ctx.stats()
+        assert (res[precision]["status"] == 0).all()
+        if precision == "refined":
+            print("  re-solved in FP64:", st["n_fp64_resolved"], flush=True)
+            assert st["n_fp64_resolved"] >= 2
+    for k in ("chi_d", "chi_m", "sigma_nl"):
+        e_or = rel_err(res["fp64"][k], ref[k])
+        e_rf = rel_err(res["refined"][k], res["fp64"][k])
+        print(f"  {k}: fp64 vs oracle {e_or:.1e}, refined vs fp64 {e_rf:.1e}", flush=True)
+        assert e_or < 1e-6 and e_rf < 1e-6, k
