@@ -17,7 +17,7 @@ ADGA_PREC_FP64, ADGA_PREC_REFINED = 0, 1
 # every symbol include/adga_amd.h declares
 EXPORTS = ["adga_ctx_create", "adga_set_option", "adga_set_local_omega", "adga_run_points", "adga_get_sigma",
            "adga_reset_sigma", "adga_get_stats", "adga_last_error", "adga_ctx_destroy", "adga_inverse_matrix_z",
-           "adga_get_giw", "adga_get_nfock", "adga_get_sigma_g_loc", "adga_get_ndga"]
+           "adga_get_giw", "adga_get_nfock", "adga_get_sigma_g_loc", "adga_get_ndga", "adga_set_vq"]
 
 
 class adga_dims(C.Structure):
@@ -38,6 +38,7 @@ def _load():
     sig = {
         "adga_ctx_create": ([C.POINTER(adga_dims), _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)], C.c_int),
         "adga_set_option": ([_vp, C.c_char_p, C.c_double], C.c_int),
+        "adga_set_vq": ([_vp, _dp], C.c_int),
         "adga_set_local_omega": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "adga_run_points": ([_vp, C.c_int, _ip, C.c_int, _dp, _dp, _dp, _ip], C.c_int),
         "adga_get_sigma": ([_vp, _dp, _dp, _dp], C.c_int),

@@ -69,6 +69,8 @@ class Context:
             raise AdgaError(rc, "adga_ctx_create failed")
         self.h = h
         self._keep = None
+        if getattr(inp, "v_q", None) is not None:
+            self.set_vq(inp.v_q)
         if do_eom is None:
             do_eom = getattr(wl, "do_eom", True)  # the reference runs a q path without the EOM
         for key, v in (("batch", batch), ("refine_tol", refine_tol), ("refine_maxit", refine_maxit),
@@ -105,6 +107,17 @@ class Context:
         self.close()
 
     # -----------------------------------------------------------------
+    def set_vq(self, v_q):
+        """V(q) for every q of the list, (n2, n2, nqp) as read_vq returns it per point
+        (src/interaction_module.f90:39-123); None removes it (do_vq = .false.)."""
+        if v_q is None:
+            self._check(lib.adga_set_vq(self.h, None))
+            return
+        v = _zf(v_q)
+        if v.shape != (self.n2, self.n2, self.nqp):
+            raise ValueError(f"v_q must be ({self.n2}, {self.n2}, {self.nqp})")
+        self._check(lib.adga_set_vq(self.h, _dptr(v)))
+
     def set_local_omega(self, iwb, chi_loc_dens, chi_loc_magn):
         n2 = self.n2
         xd, xm = _zf(chi_loc_dens), _zf(chi_loc_magn)

@@ -26,6 +26,8 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s);
 size_t lu_inv_elems(int N);
 std::vector<int> lu_ops_factor(int N, int W);
+// the same list written into buf (at most cap ints); returns the number of ints the list needs
+size_t lu_ops_factor_into(int N, int W, int* buf, size_t cap);
 // Events around the launches of the largest GEMM tile shape (the factorisation's dominant kernel):
 // mark() records one event; collect() (after the stream has completed) returns the summed
 // milliseconds of the bracketed launches and resets.
@@ -39,17 +41,39 @@ struct LuTimer {
   ~LuTimer();
 };
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
+bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, size_t nops, cplx<T>* M, int* piv, cplx<T>* inv,
                int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
                hipEvent_t lag = nullptr, hipEvent_t join = nullptr);
+template <typename T>
+inline bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv,
+                      cplx<T>* inv, int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
+                      hipEvent_t lag = nullptr, hipEvent_t join = nullptr) {
+  return launch_lu<T>(nmat, N, W, ldm, ops.data(), ops.size(), M, piv, inv, info, s, timer, s2, lag, join);
+}
 // `active` (may be null): per-matrix flags; matrices with a zero flag are skipped (Y left as is)
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active = nullptr);
+// The non-local interaction's rank-n2 term of the density BSE matrix (main.F90:710-719): W holds, per
+// point, the N x n2 rows (2/beta^2) chi0^q(nu) V(q); opgd = 1 + gamma^w_d; Q is n2 x n2 per point of scratch.
+struct VqTerm {
+  const z64* W;
+  const z64* opgd;
+  z64* Q;
+};
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
-                const z64* opgm, z64* Z, z64* B, hipStream_t s);
+                const z64* opgm, z64* Z, z64* B, hipStream_t s, const VqTerm* vq = nullptr);
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
-                     const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s);
+                     const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s, const VqTerm* vq = nullptr);
+// V(q) (vq.hip): W of a batch (qpos: 0-based q list positions, vq: (n2, n2, nqp)); the density matrices'
+// term; Out_{2b} += (1 + gamma^w_d)^T (W^T X) with X = (1 + gamma^w_d)^T (Y null) or the density solution Y
+void launch_vq_w(const Geom& g, int npts, const int* qpos, const z64* vq, const z64* chi0q, z64* W, hipStream_t s);
+template <typename T>
+void launch_vq_build(const Geom& g, int npts, int ldm, const VqTerm& v, cplx<T>* M, hipStream_t s);
+void launch_vq_add(const Geom& g, int npts, const VqTerm& v, const z64* Y, z64* Out, hipStream_t s);
+void launch_eom_fock(const Geom& g, int npts, const int* qpos, const int* qidx, const int* keom, const z64* vq,
+                     const z64* nfock, z64* shf, hipStream_t s);
+void launch_eom_hartree_v(const Geom& g, const z64* v, const z64* ndmft, z64* shf, hipStream_t s);
 void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
                     const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
                     z64* chi_out, z64* bub_out, hipStream_t s);
@@ -60,8 +84,18 @@ void launch_gather(int k, long len, const int* sel, const z64* src, z64* dst, bo
 void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
                       const z64* ndmft, z64* sdmft, hipStream_t s);
 void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s);
+// the local Hartree term alone (eom_module.f90:52-70)
+void launch_eom_static_local(const Geom& g, const z64* u, const z64* ndmft, z64* sdmft, hipStream_t s);
+// module-level kernels (modules.hip)
+void launch_chi0_loc_diag(const Geom& g, int iwb, const z64* giw, z64* w, z64* winv, hipStream_t s);
+bool launch_gkiw(const Geom& g, int k, int iw, const z64* hk, const z64* siw, const z64* dc, const z64* skiw,
+                 z64* out, hipStream_t s);
+void launch_chi_qw(const Geom& g, const z64* X, const z64* Y, z64* chi, hipStream_t s);
+void launch_ndmft(const Geom& g, const z64* giw, z64* nd, hipStream_t s);
+void launch_add_siw(const Geom& g, int nk, const z64* siw, z64* ss, hipStream_t s);
+// with vq non-null: V(q_b) = vq + qpos[b] n4 enters the density eta term and -V gamma^w_d / (beta nqp) is added
 void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
-                 hipStream_t s);
+                 hipStream_t s, const int* qpos = nullptr, const z64* vq = nullptr, const z64* gamd = nullptr);
 void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,
                        const z64* gt, z64* sigma, hipStream_t s);
 // k-grid convolutions (kconv.hip)

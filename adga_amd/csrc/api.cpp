@@ -1,131 +1,13 @@
 // C ABI (include/adga_amd.h) over the gfx950 kernels: context, device buffers, batching,
 // refinement loop and error reporting.  No exception crosses the extern "C" boundary.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
 #include <cmath>
-#include <cstdio>
 #include <cstring>
-#include <exception>
-#include <new>
-#include <string>
-#include <vector>
 
-#include "adga_amd.h"
-#include "adga_kernels.h"
+#include "ctx.h"
 
-using adga::c32;
-using adga::Geom;
-using adga::z64;
-
-constexpr int ADGA_NSTATS = 16;  // values adga_get_stats reports
-
-// Buffers of the FP64 re-solve of points whose complex64 refinement did not converge: the points'
-// S and right-hand sides are gathered into these, solved in complex128 and the solutions scattered back.
-struct FallbackBufs {
-  int cap = 0;  // points
-  z64 *S = nullptr, *B = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *M = nullptr, *inv = nullptr;
-  double* rel = nullptr;
-  int *piv = nullptr, *info = nullptr, *sel = nullptr;
-};
-
-struct adga_ctx {
-  Geom g{};
-  hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
-  hipEvent_t ev_lag = nullptr, ev_join = nullptr;
-  int lu_split = 2;                        // option "lu_streams": 2 splits each LU batch over two streams
-  hipEvent_t ev[7] = {}, ev_lu[2] = {};    // stage and LU timing of adga_run_points
-  std::string err;
-  int refine_maxit = 12;
-  double refine_tol = 1e-12;
-  bool fallback = true;  // option "fallback": unconverged complex64 solves are solved again in complex128
-  int batch = 256;
-  int ldm = 0;
-  // one-particle data (device)
-  z64 *hk = nullptr, *siw = nullptr, *giw = nullptr, *dc = nullptr, *u = nullptr, *ut = nullptr, *ndmft = nullptr;
-  z64* gt = nullptr;
-  int* keom = nullptr;
-  std::vector<int> q0;  // 0-based k index of each q (host)
-  // current bosonic frequency
-  bool have_omega = false;
-  int iwb = 0;
-  z64 *xd = nullptr, *xm = nullptr, *cd = nullptr, *cm = nullptr, *gamd = nullptr, *gamm = nullptr;
-  z64 *opgd = nullptr, *opgm = nullptr, *winv = nullptr, *chi0w = nullptr;
-  z64 *chiloc_d = nullptr, *chiloc_m = nullptr, *bubloc = nullptr;
-  // batch workspace
-  int cap = 0;
-  z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
-      *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
-  void* M = nullptr;
-  void* inv = nullptr;  // per-matrix panel inverses of the LU
-  double* rel = nullptr;
-  int *piv = nullptr, *info = nullptr, *qidx = nullptr;
-  int *act = nullptr, *keep = nullptr;  // per-matrix refinement flags, per-point "enters the self-energy" flags
-  FallbackBufs fb;
-  // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
-  std::vector<int> ops_f[2];  // LU operation lists (host) per precision
-  // accumulated self-energies
-  z64 *sig_nl = nullptr, *sig_dmft = nullptr;
-  // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
-  int bubble_mode = 2;  // 0 direct, 1 DFT, 2 DFT when the batch has >= 64 points or the cache is valid
-  int eom_mode = 1;     // 0 direct, 1 DFT (accumulated per omega, flushed on omega change / get_sigma)
-  bool do_eom = true;   // option "do_eom": the reference's do_eom (off for a q path, main.F90:341-347)
-  z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
-  bool bub_valid = false;
-  int bub_iwb = 0;
-  bool eom_pending = false;
-  int eom_iwb = 0;
-  double stats[ADGA_NSTATS] = {0};  // see adga_get_stats
-  adga::LuTimer lut;       // events around the dominant GEMM kernel's launches
-};
+using namespace adga_host;
 
 namespace {
-
-struct HipError {
-  hipError_t e;
-  const char* what;
-};
-
-#define CK(x)                                   \
-  do {                                          \
-    hipError_t e_ = (x);                        \
-    if (e_ != hipSuccess) throw HipError{e_, #x}; \
-  } while (0)
-
-template <typename F>
-int guarded(adga_ctx* c, F&& f) {
-  try {
-    return f();
-  } catch (const HipError& he) {
-    char buf[512];
-    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)he.e, hipGetErrorString(he.e), he.what);
-    if (c) c->err = buf;
-    return he.e == hipErrorOutOfMemory ? ADGA_E_NOMEM : ADGA_E_HIP;
-  } catch (const std::bad_alloc&) {
-    if (c) c->err = "host allocation failed";
-    return ADGA_E_NOMEM;
-  } catch (const std::exception& ex) {
-    if (c) c->err = std::string("host exception: ") + ex.what();
-    return ADGA_E_HIP;
-  } catch (...) {
-    if (c) c->err = "unexpected host exception";
-    return ADGA_E_HIP;
-  }
-}
-
-template <typename T>
-void dalloc(T** p, size_t count) {
-  CK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
-}
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-void upload(z64* d, const double* h, size_t count, hipStream_t s) {
-  CK(hipMemcpyAsync(d, h, count * sizeof(z64), hipMemcpyHostToDevice, s));
-}
 
 void free_batch(adga_ctx* c) {
   dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
@@ -137,6 +19,7 @@ void free_batch(adga_ctx* c) {
 void free_fallback(adga_ctx* c) {
   FallbackBufs& f = c->fb;
   dfree(f.S); dfree(f.B); dfree(f.Y); dfree(f.Z); dfree(f.R); dfree(f.M); dfree(f.inv); dfree(f.rel);
+  dfree(f.W); dfree(f.Q);
   dfree(f.piv); dfree(f.info); dfree(f.sel);
   f.cap = 0;
 }
@@ -169,9 +52,15 @@ void ensure_batch(adga_ctx* c, int nb) {
   c->cap = nb;
 }
 
-int fail(adga_ctx* c, int code, const char* msg) {
-  if (c) c->err = msg;
-  return code;
+// V(q) workspace of a batch of nb points
+void ensure_vq_batch(adga_ctx* c, int nb) {
+  if (nb <= c->vq_cap) return;
+  dfree(c->W); dfree(c->vqQ); dfree(c->qpos);
+  const Geom& g = c->g;
+  dalloc(&c->W, (size_t)nb * g.N * g.n2);
+  dalloc(&c->vqQ, (size_t)nb * g.n4);
+  dalloc(&c->qpos, nb);
+  c->vq_cap = nb;
 }
 
 void ensure_conv(adga_ctx* c) {
@@ -242,6 +131,10 @@ int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>&
     dalloc(&f.sel, cap);
     f.cap = cap;
   }
+  if (c->vq && !f.W) {
+    dalloc(&f.W, (size_t)f.cap * g.N * g.n2);
+    dalloc(&f.Q, (size_t)f.cap * g.n4);
+  }
   const long slen = (long)g.nf * g.n4, blen = 2L * g.N * g.n2;
   std::vector<double> r;
   std::vector<int> inf;
@@ -252,10 +145,15 @@ int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>&
     adga::launch_gather(k, slen, f.sel, c->S, f.S, false, s);
     adga::launch_gather(k, blen, f.sel, c->Bp, f.B, false, s);
     adga::launch_build<double>(g, k, c->ldm, f.S, c->cd, c->cm, f.M, s);
+    const adga::VqTerm fv{f.W, c->opgd, f.Q};
+    if (c->vq) {
+      adga::launch_gather(k, (long)g.N * g.n2, f.sel, c->W, f.W, false, s);
+      adga::launch_vq_build<double>(g, k, c->ldm, fv, f.M, s);
+    }
     if (!adga::launch_lu<double>(2 * k, g.N, g.N, c->ldm, c->ops_f[0], f.M, f.piv, f.inv, f.info, s) ||
         !adga::launch_lu_solve<double>(2 * k, g.N, g.n2, c->ldm, f.M, f.piv, f.inv, f.B, f.Y, false, s))
       throw HipError{hipErrorInvalidValue, "FP64 re-solve: BSE size outside the LU kernel's range"};
-    adga::launch_residual(g, k, f.S, c->cd, c->cm, f.B, f.Y, f.Z, f.R, f.rel, s);
+    adga::launch_residual(g, k, f.S, c->cd, c->cm, f.B, f.Y, f.Z, f.R, f.rel, s, c->vq ? &fv : nullptr);
     adga::launch_gather(k, blen, f.sel, f.Y, c->Y, true, s);
     r.resize(2 * k);
     inf.resize(2 * k);
@@ -290,8 +188,9 @@ int refine(adga_ctx* c, int nb, std::vector<double>& relh, std::vector<int>& inf
   std::vector<int> state(nmat, ACTIVE), act(nmat);
   std::vector<double> prev(nmat, INFINITY);
   int iters = 0;
+  const adga::VqTerm vt{c->W, c->opgd, c->vqQ};
   for (int it = 0;; ++it) {
-    adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
+    adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s, c->vq ? &vt : nullptr);
     CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     int nact = 0;
@@ -324,26 +223,6 @@ int refine(adga_ctx* c, int nb, std::vector<double>& relh, std::vector<int>& inf
   return iters;
 }
 
-}  // namespace
-
-namespace {
-// device buffer released on every exit path
-template <typename T = z64>
-struct DevBuf {
-  T* p = nullptr;
-  explicit DevBuf(size_t count) { dalloc(&p, count); }
-  ~DevBuf() { dfree(p); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-};
-int op_finish(adga_ctx* c, bool launched, const char* what) {
-  if (!launched) {
-    c->err = std::string(what) + ": orbital count above 8";
-    return ADGA_E_ARG;
-  }
-  CK(hipGetLastError());
-  return ADGA_OK;
-}
 }  // namespace
 
 extern "C" {
@@ -419,6 +298,8 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     dalloc(&c->chiloc_d, (size_t)g.n4); dalloc(&c->chiloc_m, (size_t)g.n4); dalloc(&c->bubloc, (size_t)g.n4);
     dalloc(&c->sig_nl, (size_t)g.n2 * g.nf * g.nkp_eom);
     dalloc(&c->sig_dmft, (size_t)g.n2 * g.nf);
+    dalloc(&c->sig_hf, (size_t)g.n2 * g.nkp_eom);
+    CK(hipMemsetAsync(c->sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom, s));
     CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, s));
     CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, s));
     CK(hipStreamSynchronize(s));
@@ -483,6 +364,23 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
   return fail(c, ADGA_E_ARG, "unknown option");
 }
 
+int adga_set_vq(adga_ctx* c, const double* v_q) {
+  if (!c) return ADGA_E_ARG;
+  if (c->eom_pending) return fail(c, ADGA_E_STATE, "V(q) cannot change while an omega is pending (call adga_get_sigma)");
+  return guarded(c, [&]() -> int {
+    const Geom& g = c->g;
+    if (!v_q) {
+      dfree(c->vq);
+      return ADGA_OK;
+    }
+    const size_t nv = (size_t)g.n4 * g.nqp;
+    if (!c->vq) dalloc(&c->vq, nv);
+    upload(c->vq, v_q, nv, c->stream);
+    CK(hipStreamSynchronize(c->stream));
+    return ADGA_OK;
+  });
+}
+
 int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* xm, double* chid_out,
                          double* chim_out, double* bubloc_out) {
   if (!c || !xd || !xm) return ADGA_E_ARG;
@@ -524,7 +422,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
     c->lut.flops = 0.0;
     int worst = ADGA_OK;
     hipEvent_t* ev = c->ev;
-    std::vector<int> qh, infoh, st, keep, resolved;
+    std::vector<int> qh, qp, infoh, st, keep, resolved;
     std::vector<double> relh;
     std::vector<z64> chih, bubh;
     for (int p0 = 0; p0 < npts; p0 += c->batch) {
@@ -533,12 +431,27 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       qh.resize(nb);
       for (int b = 0; b < nb; ++b) qh[b] = c->q0[iq[p0 + b] - 1];
       CK(hipMemcpyAsync(c->qidx, qh.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+      if (c->vq) {
+        ensure_vq_batch(c, nb);
+        qp.resize(nb);
+        for (int b = 0; b < nb; ++b) qp[b] = iq[p0 + b] - 1;
+        CK(hipMemcpyAsync(c->qpos, qp.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+      }
+      const adga::VqTerm vt{c->W, c->opgd, c->vqQ};
+      const adga::VqTerm* vtp = c->vq ? &vt : nullptr;
       const int nmat = 2 * nb;
       CK(hipEventRecord(ev[0], s));
       // local EOM for points at the Gamma point (main.F90:635-639)
       for (int b = 0; b < nb && c->do_eom; ++b)
-        if (iq[p0 + b] == 1)
+        if (iq[p0 + b] == 1) {
           adga::launch_eom_local(g, c->iwb, c->iwb == 0, c->u, c->gamd, c->giw, c->ndmft, c->sig_dmft, s);
+          if (c->vq && c->iwb == 0) adga::launch_eom_hartree_v(g, c->vq, c->ndmft, c->sig_hf, s);
+        }
+      if (c->vq && c->do_eom && c->iwb == 0) {
+        // non-local Fock term of every point at omega = 0 (calc_eom_static, main.F90:635-637)
+        ensure_nfock(c);
+        adga::launch_eom_fock(g, nb, c->qpos, c->qidx, c->keom, c->vq, c->nfock, c->sig_hf, s);
+      }
       CK(hipEventRecord(ev[1], s));
       const bool use_dft = c->bubble_mode == 1 || (c->bubble_mode == 2 && ((c->bub_valid && c->bub_iwb == c->iwb) || nb >= 64));
       if (use_dft) {
@@ -558,8 +471,9 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->chi0q, s);
       }
       adga::launch_smat(g, nb, c->chi0q, c->winv, c->S, s);
+      if (vtp) adga::launch_vq_w(g, nb, c->qpos, c->vq, c->chi0q, c->W, s);
       adga::launch_gamq(g, nb, c->S, c->cd, c->Z, c->gamq, s);
-      adga::launch_rhs(g, nb, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Z, c->Bp, s);
+      adga::launch_rhs(g, nb, c->S, c->cd, c->cm, c->opgd, c->opgm, c->Z, c->Bp, s, vtp);
       CK(hipEventRecord(ev[2], s));
       int iters = 0;
       double maxrel = 0.0;
@@ -570,6 +484,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       resolved.assign(nmat, 0);
       if (precision == ADGA_PREC_FP64) {
         adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, (z64*)c->M, s);
+        if (vtp) adga::launch_vq_build<double>(g, nb, c->ldm, vt, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, c->ops_f[0], (z64*)c->M, c->piv,
@@ -581,13 +496,14 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
                                                 c->Bp, c->Y, false, s);
         if (lu_ok) {
           // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
-          adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s);
+          adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s, vtp);
           CK(hipMemcpyAsync(relh.data(), c->rel, sizeof(double) * nmat, hipMemcpyDeviceToHost, s));
           CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
           CK(hipStreamSynchronize(s));
         }
       } else {
         adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, (c32*)c->M, s);
+        if (vtp) adga::launch_vq_build<float>(g, nb, c->ldm, vt, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, c->ops_f[1], (c32*)c->M, c->piv,
@@ -628,7 +544,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
                            c->chi_out, c->bub_out, s);
       CK(hipEventRecord(ev[5], s));
       if (c->do_eom) {
-        adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s);
+        adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s, c->qpos, c->vq, c->gamd);
         if (nfailed > 0) {
           CK(hipMemcpyAsync(c->keep, keep.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
           adga::launch_zero_points(nb, (long)n2 * g.N, c->keep, c->mt, s);
@@ -701,8 +617,8 @@ int adga_get_sigma(adga_ctx* c, double* sig_nl, double* sig_dmft, double* sig_hf
     CK(hipGetLastError());
     if (sig_nl) CK(hipMemcpyAsync(sig_nl, c->sig_nl, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, hipMemcpyDeviceToHost, s));
     if (sig_dmft) CK(hipMemcpyAsync(sig_dmft, c->sig_dmft, sizeof(z64) * g.n2 * g.nf, hipMemcpyDeviceToHost, s));
+    if (sig_hf) CK(hipMemcpyAsync(sig_hf, c->sig_hf, sizeof(z64) * g.n2 * g.nkp_eom, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
-    if (sig_hf) memset(sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom);
     return ADGA_OK;
   });
 }
@@ -783,6 +699,7 @@ int adga_reset_sigma(adga_ctx* c) {
     const Geom& g = c->g;
     CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, c->stream));
     CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, c->stream));
+    CK(hipMemsetAsync(c->sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom, c->stream));
     if (c->macc) CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp, c->stream));
     c->eom_pending = false;
     c->bub_valid = false;  // the next batch forms the bubble of its omega again
@@ -811,7 +728,9 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->gt); dfree(c->keom);
   dfree(c->xd); dfree(c->xm); dfree(c->cd); dfree(c->cm); dfree(c->gamd); dfree(c->gamm); dfree(c->opgd);
   dfree(c->opgm); dfree(c->winv); dfree(c->chi0w); dfree(c->chiloc_d); dfree(c->chiloc_m); dfree(c->bubloc);
-  dfree(c->sig_nl); dfree(c->sig_dmft);
+  dfree(c->sig_nl); dfree(c->sig_dmft); dfree(c->sig_hf);
+  dfree(c->vq); dfree(c->nfock); dfree(c->W); dfree(c->vqQ); dfree(c->qpos);
+  for (z64*& p : c->stage) dfree(p);
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
   for (auto e : c->ev)
     if (e) (void)hipEventDestroy(e);

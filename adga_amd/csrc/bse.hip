@@ -341,24 +341,28 @@ void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z
 }
 
 // Right-hand side of the eta system: eta A = (1+gamma^w)(1 - A) = (1+gamma^w) blockdiag(S) C, i.e.
-// B = M eta^T with B = C^T (blockdiag(S)^T (1+gamma^w)^T), in FP64.
+// B = M eta^T with B = C^T (blockdiag(S)^T (1+gamma^w)^T), in FP64; with V(q) the density channel adds
+// (1+gamma^w_d)^T (W^T (1+gamma^w_d)^T).
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
-                const z64* opgm, z64* Z, z64* B, hipStream_t s) {
+                const z64* opgm, z64* Z, z64* B, hipStream_t s, const VqTerm* vq) {
   int nmat = 2 * npts;
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, nullptr, opgd,
                      opgm, Z);
   resid_gemm(g, npts, 0, 2, cd, cm, Z, B, nullptr, nullptr, s);
+  if (vq) launch_vq_add(g, npts, *vq, nullptr, B, s);
 }
 
-// R = B - M Y with M Y = Y - C^T (blockdiag(S)^T Y), in FP64 from the unrounded factors S and C.
+// R = B - M Y with M Y = Y - C^T (blockdiag(S)^T Y), in FP64 from the unrounded factors S and C (with V(q):
+// minus (1 + gamma^w_d)^T (W^T Y) in the density channel).
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
-                     const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s) {
+                     const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s, const VqTerm* vq) {
   int nmat = 2 * npts;
   long tot = (long)nmat * g.N * g.n2;
   hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Y, nullptr,
                      nullptr, Z);
   resid_gemm(g, npts, 0, 2, cd, cm, Z, R, B, Y, s);
+  if (vq) launch_vq_add(g, npts, *vq, Y, R, s);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);
 }
 

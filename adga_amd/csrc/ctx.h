@@ -1,0 +1,169 @@
+// Host-side context of the C ABI (include/adga_amd.h) and the helpers its entry points share:
+// HIP error checks that throw inside `guarded`, which turns them into status codes at the boundary.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "adga_amd.h"
+#include "adga_kernels.h"
+
+using adga::c32;
+using adga::Geom;
+using adga::z64;
+
+constexpr int ADGA_NSTATS = 16;  // values adga_get_stats reports
+
+// Buffers of the FP64 re-solve of points whose complex64 refinement did not converge: the points'
+// S and right-hand sides are gathered into these, solved in complex128 and the solutions scattered back.
+struct FallbackBufs {
+  int cap = 0;  // points
+  z64 *S = nullptr, *B = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *M = nullptr, *inv = nullptr;
+  z64 *W = nullptr, *Q = nullptr;  // V(q) rows and products of the gathered points
+  double* rel = nullptr;
+  int *piv = nullptr, *info = nullptr, *sel = nullptr;
+};
+
+struct adga_ctx {
+  Geom g{};
+  hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
+  hipEvent_t ev_lag = nullptr, ev_join = nullptr;
+  int lu_split = 2;                        // option "lu_streams": 2 splits each LU batch over two streams
+  hipEvent_t ev[7] = {}, ev_lu[2] = {};    // stage and LU timing of adga_run_points
+  std::string err;
+  int refine_maxit = 12;
+  double refine_tol = 1e-12;
+  bool fallback = true;  // option "fallback": unconverged complex64 solves are solved again in complex128
+  int batch = 256;
+  int ldm = 0;
+  // one-particle data (device)
+  z64 *hk = nullptr, *siw = nullptr, *giw = nullptr, *dc = nullptr, *u = nullptr, *ut = nullptr, *ndmft = nullptr;
+  z64* gt = nullptr;
+  int* keom = nullptr;
+  std::vector<int> q0;  // 0-based k index of each q (host)
+  // non-local interaction V(q) (adga_set_vq): (n2, n2, nqp) on the device, n_fock for its Fock term,
+  // the batch's q list positions, V rows W and n2 x n2 products
+  z64 *vq = nullptr, *nfock = nullptr, *W = nullptr, *vqQ = nullptr;
+  int* qpos = nullptr;
+  int vq_cap = 0;
+  // current bosonic frequency
+  bool have_omega = false;
+  int iwb = 0;
+  z64 *xd = nullptr, *xm = nullptr, *cd = nullptr, *cm = nullptr, *gamd = nullptr, *gamm = nullptr;
+  z64 *opgd = nullptr, *opgm = nullptr, *winv = nullptr, *chi0w = nullptr;
+  z64 *chiloc_d = nullptr, *chiloc_m = nullptr, *bubloc = nullptr;
+  // batch workspace
+  int cap = 0;
+  z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
+      *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
+  void* M = nullptr;
+  void* inv = nullptr;  // per-matrix panel inverses of the LU
+  double* rel = nullptr;
+  int *piv = nullptr, *info = nullptr, *qidx = nullptr;
+  int *act = nullptr, *keep = nullptr;  // per-matrix refinement flags, per-point "enters the self-energy" flags
+  FallbackBufs fb;
+  // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
+  std::vector<int> ops_f[2];  // LU operation lists (host) per precision
+  // accumulated self-energies
+  z64 *sig_nl = nullptr, *sig_dmft = nullptr, *sig_hf = nullptr;
+  // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
+  int bubble_mode = 2;  // 0 direct, 1 DFT, 2 DFT when the batch has >= 64 points or the cache is valid
+  int eom_mode = 1;     // 0 direct, 1 DFT (accumulated per omega, flushed on omega change / get_sigma)
+  bool do_eom = true;   // option "do_eom": the reference's do_eom (off for a q path, main.F90:341-347)
+  z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
+  bool bub_valid = false;
+  int bub_iwb = 0;
+  bool eom_pending = false;
+  int eom_iwb = 0;
+  double stats[ADGA_NSTATS] = {0};  // see adga_get_stats
+  // device staging of caller arrays for the module-level entry points (api_modules.cpp)
+  z64* stage[6] = {};
+  size_t stage_n[6] = {};
+  adga::LuTimer lut;       // events around the dominant GEMM kernel's launches
+};
+
+namespace adga_host {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define CK(x)                                   \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) throw HipError{e_, #x}; \
+  } while (0)
+
+template <typename F>
+inline int guarded(adga_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (const HipError& he) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)he.e, hipGetErrorString(he.e), he.what);
+    if (c) c->err = buf;
+    return he.e == hipErrorOutOfMemory ? ADGA_E_NOMEM : ADGA_E_HIP;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return ADGA_E_NOMEM;
+  } catch (const std::exception& ex) {
+    if (c) c->err = std::string("host exception: ") + ex.what();
+    return ADGA_E_HIP;
+  } catch (...) {
+    if (c) c->err = "unexpected host exception";
+    return ADGA_E_HIP;
+  }
+}
+
+template <typename T>
+void dalloc(T** p, size_t count) {
+  CK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+inline void upload(z64* d, const double* h, size_t count, hipStream_t s) {
+  CK(hipMemcpyAsync(d, h, count * sizeof(z64), hipMemcpyHostToDevice, s));
+}
+
+inline int fail(adga_ctx* c, int code, const char* msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+// device buffer released on every exit path
+template <typename T = z64>
+struct DevBuf {
+  T* p = nullptr;
+  explicit DevBuf(size_t count) { dalloc(&p, count); }
+  ~DevBuf() { dfree(p); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+// n_fock of the context's one-particle data (get_nfock), formed on first use
+inline void ensure_nfock(adga_ctx* c) {
+  if (c->nfock) return;
+  dalloc(&c->nfock, (size_t)c->g.nkp * c->g.n2);
+  if (!adga::launch_op_nfock(c->g, c->hk, c->siw, c->dc, c->nfock, c->stream))
+    throw HipError{hipErrorInvalidValue, "n_fock: orbital count above 8"};
+}
+
+inline int op_finish(adga_ctx* c, bool launched, const char* what) {
+  if (!launched) {
+    c->err = std::string(what) + ": orbital count above 8";
+    return ADGA_E_ARG;
+  }
+  CK(hipGetLastError());
+  return ADGA_OK;
+}
+
+}  // namespace adga_host

@@ -49,10 +49,12 @@ __global__ void k_chi(Geom g, int npts, const z64* __restrict__ chi0q, const z64
   if (ch == 0) bub_out[(long)b * g.n4 + e] = csub(cscale(bs, ib2), bubloc[e]);
 }
 
-// m_tot = -(U - U~/2) eta_d /(beta nqp) + 1.5 U~ eta_m /(beta nqp) + U gamma^q_d /(beta nqp)
-// (eom_module.f90:187-201; no V(q)).  Output (n2, N) Fortran per point.
+// m_tot = -(U + V - U~/2) eta_d /(beta nqp) + 1.5 U~ eta_m /(beta nqp) + U gamma^q_d /(beta nqp)
+//        - V gamma^w_d /(beta nqp)   (eom_module.f90:183-207; the V terms only with V(q): vq non-null).
+// Output (n2, N) Fortran per point.
 __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* __restrict__ ut,
-                       const z64* __restrict__ eta, const z64* __restrict__ gamq, z64* __restrict__ mt) {
+                       const z64* __restrict__ eta, const z64* __restrict__ gamq, z64* __restrict__ mt,
+                       const int* __restrict__ qpos, const z64* __restrict__ vq, const z64* __restrict__ gamd) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long per = (long)g.n2 * g.N;
   if (idx >= per * npts) return;
@@ -63,10 +65,16 @@ __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* _
   const z64* ed = eta + (long)(2 * b) * per;
   const z64* em = eta + (long)(2 * b + 1) * per;
   const z64* gq = gamq + (long)(2 * b) * per;  // residual layout: point b at slot 2b
-  z64 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0);
+  const z64* v = vq ? vq + (long)qpos[b] * g.n4 : nullptr;
+  z64 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0), t4 = mk(0.0, 0.0);
   for (int a = 0; a < g.n2; ++a) {
     z64 uu = u[s + a * g.n2], uw = ut[s + a * g.n2];
     z64 uwork = mk(uu.re - 0.5 * uw.re, uu.im - 0.5 * uw.im);
+    if (v) {
+      const z64 va = v[s + a * g.n2];
+      uwork = cadd(va, uwork);
+      t4 = cfma(va, gamd[a + x * g.n2], t4);
+    }
     t1 = cfma(uwork, ed[a + x * g.n2], t1);
     t2 = cfma(uw, em[a + x * g.n2], t2);
     t3 = cfma(uu, gq[a + x * g.n2], t3);
@@ -75,6 +83,7 @@ __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* _
   z64 m = cscale(t1, a1);
   m = cadd(cscale(t2, a2), m);
   m = cadd(cscale(t3, a3), m);
+  if (v) m = cadd(cscale(t4, a1), m);
   mt[idx] = m;
 }
 
@@ -170,9 +179,10 @@ void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, cons
 }
 
 void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
-                 hipStream_t s) {
+                 hipStream_t s, const int* qpos, const z64* vq, const z64* gamd) {
   long tot = (long)npts * g.n2 * g.N;
-  hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt);
+  hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt,
+                     qpos, vq, gamd);
 }
 
 void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,
@@ -180,6 +190,10 @@ void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const 
   long te = (long)g.n2 * g.iwf * g.nkp_eom;
   hipLaunchKernelGGL(k_eom, dim3((unsigned)((te + 255) / 256)), dim3(256), 0, s, g, iwb, npts, qidx, keom, mt, gt,
                      sigma);
+}
+
+void launch_eom_static_local(const Geom& g, const z64* u, const z64* ndmft, z64* sdmft, hipStream_t s) {
+  hipLaunchKernelGGL(k_eom_static, dim3((g.n * g.nf + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
 }
 
 void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,

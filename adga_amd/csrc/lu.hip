@@ -738,11 +738,18 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ld
 // ---------------------------------------------------------------- host: operation lists
 namespace {
 
+// operation list: a growing vector, or (buf non-null) a caller's fixed array of cap ints, in which
+// case n still counts every int the list needs
 struct Ops {
   std::vector<int> v;
+  int* buf = nullptr;
+  size_t cap = 0, n = 0;
   void add(int ty, int a, int b, int c, int d, int e = 0, int f = 0) {
     int o[8] = {ty, a, b, c, d, e, f, 0};
-    v.insert(v.end(), o, o + 8);
+    if (!buf) v.insert(v.end(), o, o + 8);
+    else if (n + 8 <= cap)
+      for (int i = 0; i < 8; ++i) buf[n + i] = o[i];
+    n += 8;
   }
 };
 
@@ -824,6 +831,15 @@ std::vector<int> lu_ops_factor(int N, int W) {
   rec_lu(ops, N, W, 0, N, true);
   if (W > N) rec_trsm_u(ops, 0, N, N, W);
   return ops.v;
+}
+
+size_t lu_ops_factor_into(int N, int W, int* buf, size_t cap) {
+  Ops ops;
+  ops.buf = buf;
+  ops.cap = cap;
+  rec_lu(ops, N, W, 0, N, true);
+  if (W > N) rec_trsm_u(ops, 0, N, N, W);
+  return ops.n;
 }
 
 namespace {
@@ -957,7 +973,7 @@ bool lu_op(int nmat, int N, int W, int ldm, const int* op, cplx<T>* M, int* piv,
 // fork, so they own the machine; after each, the second stream starts one operation behind again.
 // `lag` and `join` are events the caller owns.
 template <typename T>
-bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv, cplx<T>* inv,
+bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, size_t nops, cplx<T>* M, int* piv, cplx<T>* inv,
                int* info, hipStream_t s, LuTimer* timer, hipStream_t s2, hipEvent_t lag, hipEvent_t join) {
   if (panel_cfg(N) < 0 || N > PERM_MAX_N || nmat < 1) return false;
   const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
@@ -980,7 +996,7 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
     uinv = true;
   };
   bool relag = true;  // the second stream waits for the first one's next operation
-  for (size_t o = 0; o + 8 <= ops.size(); o += 8) {
+  for (size_t o = 0; o + 8 <= nops; o += 8) {
     const int* op = &ops[o];
     if (op[0] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
     const bool big = op[0] == OP_GEMM && op[2] - op[1] > 64 && op[4] - op[3] > 64;
@@ -1006,9 +1022,9 @@ bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cpl
   }
   return true;
 }
-template bool launch_lu<float>(int, int, int, int, const std::vector<int>&, c32*, int*, c32*, int*, hipStream_t,
+template bool launch_lu<float>(int, int, int, int, const int*, size_t, c32*, int*, c32*, int*, hipStream_t,
                                LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
-template bool launch_lu<double>(int, int, int, int, const std::vector<int>&, z64*, int*, z64*, int*, hipStream_t,
+template bool launch_lu<double>(int, int, int, int, const int*, size_t, z64*, int*, z64*, int*, hipStream_t,
                                 LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
 
 void LuTimer::mark(hipStream_t s) {

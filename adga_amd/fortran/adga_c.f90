@@ -9,6 +9,7 @@ module adga_c
   public :: adga_dims, c_adga_ctx_create, c_adga_set_option, c_adga_set_local_omega, c_adga_run_points, &
             c_adga_get_sigma, c_adga_reset_sigma, c_adga_get_stats, c_adga_last_error, c_adga_ctx_destroy, &
             c_adga_inverse_matrix_z, c_adga_get_giw, c_adga_get_nfock, c_adga_get_sigma_g_loc, c_adga_get_ndga, &
+            c_adga_set_vq, &
             ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
 
   integer(c_int), parameter :: ADGA_OK = 0, ADGA_E_SINGULAR = 3
@@ -35,6 +36,12 @@ module adga_c
        character(kind=c_char), intent(in) :: key(*)
        real(c_double), value :: value
      end function c_adga_set_option
+
+     integer(c_int) function c_adga_set_vq(ctx, v_q) bind(c, name='adga_set_vq')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: v_q(*)
+     end function c_adga_set_vq
 
      integer(c_int) function c_adga_set_local_omega(ctx, iwb, chi_loc_dens, chi_loc_magn, chi_loc_dens_out, &
           chi_loc_magn_out, bubble_loc_out) bind(c, name='adga_set_local_omega')

@@ -86,6 +86,17 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
 int adga_set_option(adga_ctx *ctx, const char *key, double value);
 
 /*
+ * Non-local interaction V(q) for every q of the list: v_q(ndim^2, ndim^2, nqp) in the compound index
+ * read_vq returns (v(i1 = {k i}, i2 = {j l}) = V_ijkl(q), i.e. i1 = (i-1)*ndim + k, i2 = (l-1)*ndim + j
+ * in 1-based Fortran terms).  With V set, adga_run_points adds the density-channel BSE term
+ * -2/beta^2 chi0^q(nu) V(q) (1 + gamma^w_d), the V terms of m_tot and, at omega = 0, the non-local Hartree
+ * (at the first q of the list) and Fock terms of sigma_hf, as the reference does with do_vq.  NULL
+ * removes V.  ADGA_E_STATE while the EOM of an omega is pending (call adga_get_sigma first).
+ * Replaces: read_vq per point (src/interaction_module.f90:39-123, src/main.F90:497-503) with do_vq set.
+ */
+int adga_set_vq(adga_ctx *ctx, const double *v_q);
+
+/*
  * Local update for bosonic frequency iwb: chi0^w and its inverse, chi0^w F_r = chi_loc_r chi0^w^-1 - 1
  * after the transpose symmetrisation, gamma^w_r and 1 + gamma^w_r.
  *   chi_loc_dens/magn(N,N): what read_vertex returns (src/hdf5_module.f90:708-770).
@@ -117,7 +128,7 @@ int adga_run_points(adga_ctx *ctx, int npts, const int *iq, int precision, doubl
 /*
  * Downloads the accumulated self-energies, with Sigma(-nu-1) = Sigma(nu)^H filled in:
  *   sigma_nl(ndim,ndim,2*iwfmax_small,nkp_eom), sigma_dmft(ndim,ndim,2*iwfmax_small),
- *   sigma_hf(ndim,ndim,nkp_eom) (zero: no V(q)).  Any pointer may be NULL.
+ *   sigma_hf(ndim,ndim,nkp_eom) (zero without V(q)).  Any pointer may be NULL.
  * Replaces: reading sigma_nl/sigma_dmft/sigma_hf after the loop (src/main.F90:841-855).
  */
 int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *sigma_hf);
