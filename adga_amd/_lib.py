@@ -17,7 +17,8 @@ ADGA_PREC_FP64, ADGA_PREC_REFINED = 0, 1
 # every symbol include/adga_amd.h declares
 EXPORTS = ["adga_ctx_create", "adga_set_option", "adga_set_local_omega", "adga_run_points", "adga_get_sigma",
            "adga_reset_sigma", "adga_get_stats", "adga_last_error", "adga_ctx_destroy", "adga_inverse_matrix_z",
-           "adga_get_giw", "adga_get_nfock", "adga_get_sigma_g_loc", "adga_get_ndga", "adga_set_vq"]
+           "adga_get_giw", "adga_get_nfock", "adga_get_sigma_g_loc", "adga_get_ndga", "adga_set_vq",
+           "adga_inverse_matrix_z_noalloc"]
 
 
 class adga_dims(C.Structure):
@@ -47,6 +48,7 @@ def _load():
         "adga_last_error": ([_vp, C.c_char_p, C.c_int], C.c_int),
         "adga_ctx_destroy": ([_vp], C.c_int),
         "adga_inverse_matrix_z": ([C.c_int, _dp, _ip], C.c_int),
+        "adga_inverse_matrix_z_noalloc": ([C.c_int, _dp, _ip], C.c_int),
         "adga_get_giw": ([_vp, _dp], C.c_int),
         "adga_get_nfock": ([_vp, _dp], C.c_int),
         "adga_get_sigma_g_loc": ([_vp, _dp, _dp, _dp], C.c_int),

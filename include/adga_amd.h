@@ -200,6 +200,20 @@ int adga_ctx_destroy(adga_ctx *ctx);
  */
 int adga_inverse_matrix_z(int n, double *m, int *ierr);
 
+/* largest n adga_inverse_matrix_z_noalloc takes */
+#define ADGA_INVERSE_NOALLOC_MAX_N 1024
+
+/*
+ * adga_inverse_matrix_z without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the matrix, its right-hand
+ * sides, the panel inverses and the pivots live in one file-scope device array of the library
+ * (34.6 MB, held by each GPU the library runs on until the process ends).  Same arguments, results and
+ * errors as adga_inverse_matrix_z; n above ADGA_INVERSE_NOALLOC_MAX_N returns ADGA_E_ARG.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU race.
+ * No counterpart in the reference.
+ */
+int adga_inverse_matrix_z_noalloc(int n, double *m, int *ierr);
+
 #ifdef __cplusplus
 }
 #endif
