@@ -9,7 +9,8 @@ module adga_c
   public :: adga_dims, c_adga_ctx_create, c_adga_set_option, c_adga_set_local_omega, c_adga_run_points, &
             c_adga_get_sigma, c_adga_reset_sigma, c_adga_get_stats, c_adga_last_error, c_adga_ctx_destroy, &
             c_adga_inverse_matrix_z, c_adga_get_giw, c_adga_get_nfock, c_adga_get_sigma_g_loc, c_adga_get_ndga, &
-            c_adga_set_vq, &
+            c_adga_set_vq, c_adga_bubble_q, c_adga_chi0_loc, c_adga_chi_qw, c_adga_eom_dynamic, c_adga_eom_dmft, &
+            c_adga_eom_static, c_adga_get_gkiw, c_adga_get_ndmft, c_adga_add_siw_dmft, &
             ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
 
   integer(c_int), parameter :: ADGA_OK = 0, ADGA_E_SINGULAR = 3
@@ -124,5 +125,70 @@ module adga_c
        complex(c_double_complex), intent(in) :: sigma_sum(*)
        complex(c_double_complex), intent(out) :: n_dga_k(*), n_dga(*)
      end function c_adga_get_ndga
+
+     ! module-level entry points (one call per call of a reference module procedure)
+     integer(c_int) function c_adga_bubble_q(ctx, iwb, q_k, chi0q) bind(c, name='adga_bubble_q')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iwb, q_k
+       complex(c_double_complex), intent(out) :: chi0q(*)
+     end function c_adga_bubble_q
+
+     integer(c_int) function c_adga_chi0_loc(ctx, iwb, chi0w_diag, chi0w_inv_diag) bind(c, name='adga_chi0_loc')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iwb
+       complex(c_double_complex), intent(out) :: chi0w_diag(*), chi0w_inv_diag(*)
+     end function c_adga_chi0_loc
+
+     integer(c_int) function c_adga_chi_qw(ctx, interm3, chi0_sum, chi_qw) bind(c, name='adga_chi_qw')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: interm3(*), chi0_sum(*)
+       complex(c_double_complex), intent(inout) :: chi_qw(*)
+     end function c_adga_chi_qw
+
+     integer(c_int) function c_adga_eom_dynamic(ctx, iwb, iq, etaqd, etaqm, gammawd, gammaqd, v, sigma_nl) &
+          bind(c, name='adga_eom_dynamic')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx, v
+       integer(c_int), value :: iwb, iq
+       complex(c_double_complex), intent(in) :: etaqd(*), etaqm(*), gammawd(*), gammaqd(*)
+       complex(c_double_complex), intent(inout) :: sigma_nl(*)
+     end function c_adga_eom_dynamic
+
+     integer(c_int) function c_adga_eom_dmft(ctx, iwb, gammawd, sigma_dmft) bind(c, name='adga_eom_dmft')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iwb
+       complex(c_double_complex), intent(in) :: gammawd(*)
+       complex(c_double_complex), intent(inout) :: sigma_dmft(*)
+     end function c_adga_eom_dmft
+
+     integer(c_int) function c_adga_eom_static(ctx, iq, v, nonlocal, sigma_dmft, sigma_hf) bind(c, name='adga_eom_static')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx, v
+       integer(c_int), value :: iq, nonlocal
+       complex(c_double_complex), intent(inout) :: sigma_dmft(*), sigma_hf(*)
+     end function c_adga_eom_static
+
+     integer(c_int) function c_adga_get_gkiw(ctx, ik, iw, skiw, gkiw) bind(c, name='adga_get_gkiw')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx, skiw
+       integer(c_int), value :: ik, iw
+       complex(c_double_complex), intent(out) :: gkiw(*)
+     end function c_adga_get_gkiw
+
+     integer(c_int) function c_adga_get_ndmft(ctx, n_dmft) bind(c, name='adga_get_ndmft')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: n_dmft(*)
+     end function c_adga_get_ndmft
+
+     integer(c_int) function c_adga_add_siw_dmft(ctx, sigma_sum) bind(c, name='adga_add_siw_dmft')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(inout) :: sigma_sum(*)
+     end function c_adga_add_siw_dmft
   end interface
 end module adga_c

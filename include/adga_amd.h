@@ -133,6 +133,82 @@ int adga_run_points(adga_ctx *ctx, int npts, const int *iq, int precision, doubl
  */
 int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *sigma_hf);
 
+/*
+ * Module-level entry points: one call per call of a reference module procedure, on the context's
+ * one-particle data, for replacement Fortran modules with the reference's names and interfaces
+ * (one_particle_quant_module, susc_module, eom_module) that leave the calling program unchanged.
+ * Caller arrays are host memory in the reference's layouts; every call runs on the GPU and returns
+ * when its outputs are written.  The batched path (adga_run_points) is the fast one: these cost a
+ * host-device round trip per call.
+ */
+
+/*
+ * chi0q(ndim^2, ndim^2, 2*iwfmax_small): the bubble of the q whose k-grid index is q_k (1-based) at
+ * omega iwb, every nu of the box: -beta/nkp sum_k G(k, nu)_il G(k-q, nu-omega)_kj at
+ * [(i-1)*ndim + j, (l-1)*ndim + k], i.e. the sum of accumulate_chi0 over ik divided by nkp.
+ * Replaces: accumulate_chi0 over the k loop (src/one_particle_quant_module.f90:228-290,
+ * src/main.F90:647-653).
+ */
+int adga_bubble_q(adga_ctx *ctx, int iwb, int q_k, double *chi0q);
+
+/*
+ * Diagonals (ndim^2, 2*iwfmax_small) of chi0_loc(nu) = -beta giw(nu, i) giw(nu - omega, j) at
+ * i1 = (i-1)*ndim + j and of its inverse, for every nu of the box (either pointer may be NULL).
+ * Replaces: get_chi0_loc, get_chi0_loc_inv (src/one_particle_quant_module.f90:192-225).
+ */
+int adga_chi0_loc(adga_ctx *ctx, int iwb, double *chi0w_diag, double *chi0w_inv_diag);
+
+/*
+ * chi_qw(ndim^2, ndim^2) += beta^-2 sum_{i3, nu} interm3(:, {i3, nu}) chi0_sum(i3, :, nu), with
+ * interm3(ndim^2, maxdim) and chi0_sum(ndim^2, ndim^2, 2*iwfmax_small).
+ * Replaces: calc_chi_qw (src/susc_module.f90:34-56).
+ */
+int adga_chi_qw(adga_ctx *ctx, const double *interm3, const double *chi0_sum, double *chi_qw);
+
+/*
+ * The dynamic EOM of one point (q list position iq, 1-based, at omega iwb): m_tot from eta_d, eta_m,
+ * gamma^q_d (and V(q), gamma^w_d when v is not NULL), added into the caller's
+ * sigma_nl(ndim, ndim, 2*iwfmax_small, nkp_eom) with Sigma(-nu-1) = Sigma(nu)^H.  k - q is computed
+ * from the context's q and EOM k lists (the kq_ind_eom table index_kq_eom builds).
+ * Replaces: calc_eom_dynamic (src/eom_module.f90:165-233).
+ */
+int adga_eom_dynamic(adga_ctx *ctx, int iwb, int iq, const double *etaqd, const double *etaqm,
+                     const double *gammawd, const double *gammaqd, const double *v, double *sigma_nl);
+
+/*
+ * The local EOM with the DMFT g (default branch): sigma_dmft(ndim, ndim, 2*iwfmax_small) inout.
+ * Replaces: calc_eom_dmft (src/eom_module.f90:115-161) without the debug option Dmft_phbar.
+ */
+int adga_eom_dmft(adga_ctx *ctx, int iwb, const double *gammawd, double *sigma_dmft);
+
+/*
+ * The static EOM terms of q list position iq: at iq == 1 the local Hartree term into
+ * sigma_dmft(ndim, ndim, 2*iwfmax_small); with nonlocal and v(ndim^2, ndim^2) not NULL (do_vq) the
+ * non-local Hartree term (iq == 1) and the Fock term into sigma_hf(ndim, ndim, nkp_eom).
+ * Replaces: calc_eom_static (src/eom_module.f90:39-111).
+ */
+int adga_eom_static(adga_ctx *ctx, int iq, const double *v, int nonlocal, double *sigma_dmft, double *sigma_hf);
+
+/*
+ * gkiw(ndim, ndim) = G(k_ik, nu_iw) (ik 1-based, nu = pi (2 iw + 1) / beta) with Sigma = siw(iw, :)
+ * on the diagonal when skiw is NULL (iw in [-iwmax, iwmax)), or the caller's skiw(ndim, ndim).
+ * Replaces: get_gkiw(ik, iwf, iwb) with iw = iwf - iwb, get_gkiw_dga
+ * (src/one_particle_quant_module.f90:81-127).
+ */
+int adga_get_gkiw(adga_ctx *ctx, int ik, int iw, const double *skiw, double *gkiw);
+
+/*
+ * n_dmft(ndim) = 2/beta sum_{iw >= 0} Re giw(iw, :) + 1/2 from the context's giw; it also becomes the
+ * context's n_dmft.  Replaces: get_ndmft (src/one_particle_quant_module.f90:295-313).
+ */
+int adga_get_ndmft(adga_ctx *ctx, double *n_dmft);
+
+/*
+ * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) += delta_ab siw(nu, a).
+ * Replaces: add_siw_dmft (src/eom_module.f90:237-251).
+ */
+int adga_add_siw_dmft(adga_ctx *ctx, double *sigma_sum);
+
 /* Zeroes the accumulated self-energies and drops the per-omega bubble cache (the next batch forms the
    bubble of its omega again). */
 int adga_reset_sigma(adga_ctx *ctx);
