@@ -63,6 +63,25 @@ def build_fortran() -> str | None:
     return FCHECK
 
 
+H5_DIR = os.environ.get("ADGA_HDF5_DIR", "/opt/conda")
+H5LIB = os.path.join(HERE, "libadga_h5.so")
+
+
+def build_h5() -> str | None:
+    """HDF5 readers and writers of the reference's file formats (host C++, libadga_h5.so)."""
+    if not os.path.exists(os.path.join(H5_DIR, "include", "hdf5.h")):
+        return None
+    tmp = H5LIB + ".tmp"
+    r = subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I" + os.path.join(H5_DIR, "include"),
+                        "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "h5io.cpp"), "-o", tmp,
+                        "-L" + os.path.join(H5_DIR, "lib"), "-lhdf5", "-Wl,-rpath," + os.path.join(H5_DIR, "lib")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{r.stderr[-4000:]}")
+    os.replace(tmp, H5LIB)
+    return H5LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -74,8 +93,10 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
     fexe = build_fortran()
+    h5 = build_h5()
     if verbose:
-        print("built", LIB, "and", fexe or "(no amdflang: Fortran layer skipped)")
+        print("built", LIB, "and", fexe or "(no amdflang: Fortran layer skipped)", "and",
+              h5 or "(no HDF5 headers: HDF5 I/O skipped)")
     return LIB
 
 

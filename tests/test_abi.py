@@ -43,3 +43,13 @@ def test_argument_validation_without_gpu():
     assert lib.adga_inverse_matrix_z(0, z, C.byref(ierr)) == _lib.ADGA_E_ARG
     assert lib.adga_inverse_matrix_z(4097, z, C.byref(ierr)) == _lib.ADGA_E_ARG  # above ADGA_INVERSE_MAX_N
     assert lib.adga_ctx_destroy(None) == _lib.ADGA_OK
+
+
+def test_hdf5_library_exports_every_declared_symbol():
+    h5io = pytest.importorskip("adga_amd.h5io", reason="libadga_h5.so not built (needs the HDF5 C library)")
+    src = open(os.path.join(ROOT, "include", "adga_h5.h")).read()
+    declared = sorted(set(re.findall(r"^int\s+(adga_h5_\w+)\s*\(", src, flags=re.M)))
+    lib = C.CDLL(h5io.LIBPATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(h5io.EXPORTS) == declared
