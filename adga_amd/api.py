@@ -118,16 +118,20 @@ class Context:
             raise ValueError(f"v_q must be ({self.n2}, {self.n2}, {self.nqp})")
         self._check(lib.adga_set_vq(self.h, _dptr(v)))
 
-    def set_local_omega(self, iwb, chi_loc_dens, chi_loc_magn):
+    def set_local_omega(self, iwb, chi_loc_dens, chi_loc_magn, box=None):
+        """Local update at omega = iwb; `box` (default iwfmax_small): this omega's fermionic box
+        nu in [-box, box), with chi_loc of size n^2 * 2 * box."""
         n2 = self.n2
+        box = self.iwfmax_small if box is None else int(box)
         xd, xm = _zf(chi_loc_dens), _zf(chi_loc_magn)
-        N = n2 * self.nf
+        N = n2 * 2 * box
         if xd.shape != (N, N) or xm.shape != (N, N):
             raise ValueError(f"chi_loc must be ({N}, {N})")
         od = np.zeros((n2, n2), dtype=np.complex128, order="F")
         om = np.zeros_like(od)
         bl = np.zeros_like(od)
-        self._check(lib.adga_set_local_omega(self.h, int(iwb), _dptr(xd), _dptr(xm), _dptr(od), _dptr(om), _dptr(bl)))
+        self._check(lib.adga_set_local_omega_box(self.h, int(iwb), box, _dptr(xd), _dptr(xm), _dptr(od), _dptr(om),
+                                                 _dptr(bl)))
         return od, om, bl
 
     def run_points(self, iq, precision="fp64", raise_on_error=True):
@@ -201,11 +205,12 @@ class Context:
         return dict(zip(keys, out.tolist()))
 
 
-def run_loop(ctx: Context, points, chi_loc_fn, precision="fp64"):
+def run_loop(ctx: Context, points, chi_loc_fn, precision="fp64", box_fn=None):
     """Drive `points` = [(iwb, iq0), ...] (iq0 0-based, omega-major like main.F90:381-389) through the
     device path.  Consecutive points with the same omega form one batch.  Returns the quantities the
     reference accumulates: per-point chi_d, chi_m, bubble_nl, and sigma_nl, sigma_dmft, chi_loc_d/m and
-    bubble_loc per omega (added when the omega block starts at iq == 1, main.F90:519-577)."""
+    bubble_loc per omega (added when the omega block starts at iq == 1, main.F90:519-577).  `box_fn(iwb)`
+    (optional) gives each omega's fermionic box; chi_loc_fn then returns chi_loc of that box."""
     n2 = ctx.n2
     npts = len(points)
     out = {"chi_d": np.zeros((n2, n2, npts), dtype=np.complex128, order="F"),
@@ -219,7 +224,8 @@ def run_loop(ctx: Context, points, chi_loc_fn, precision="fp64"):
         e = p
         while e < npts and points[e][0] == iwb:
             e += 1
-        ld, lm, bl = ctx.set_local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"))
+        box = None if box_fn is None else box_fn(iwb)
+        ld, lm, bl = ctx.set_local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"), box)
         if points[p][1] == 0:
             out["chi_loc_d"][iwb] = ld
             out["chi_loc_m"][iwb] = lm

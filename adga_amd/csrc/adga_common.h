@@ -82,8 +82,9 @@ __host__ __device__ __forceinline__ cplx<D> ccast(cplx<S> a) {
 // Problem geometry shared by all kernels of one context (plain data, passed by value).
 struct Geom {
   int n, n2, n4;        // orbitals, n^2, n^4
-  int nf;               // fermionic frequencies of the box (2*iwfmax_small)
-  int iwf;              // iwfmax_small
+  int nf;               // fermionic frequencies of the BSE box of the current omega (2*iwf)
+  int iwf;              // half of it (iwfmax_small, or less where the box shrinks with |omega|)
+  int iws;              // iwfmax_small: the self-energies' frequency box (offset iws, 2*iws frequencies)
   int iwbmax;           // iwbmax_small
   int N;                // n2*nf, BSE dimension
   int nkx, nky, nkz, nkp;

@@ -88,18 +88,18 @@ void ensure_conv(adga_ctx* c) {
 
 void eom_flush(adga_ctx* c) {
   if (!c->eom_pending) return;
-  adga::launch_eom_flush(c->g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream);
+  adga::launch_eom_flush(c->eom_g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream);
   CK(hipGetLastError());
   CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)c->g.iwf * c->g.n4 * c->g.nkp, c->stream));
   c->eom_pending = false;
 }
 
-// prec 0: complex128 LU, prec 1: complex64 LU
-void ensure_ops(adga_ctx* c, int prec) {
-  if (!c->ops_f[prec].empty()) return;
-  (void)prec;
-  const int N = c->g.N;
-  c->ops_f[prec] = adga::lu_ops_factor(N, N);
+// LU operation list of the current omega's BSE size; prec 0: complex128 LU, prec 1: complex64 LU
+const std::vector<int>& ensure_ops(adga_ctx* c, int prec) {
+  const int N = c->gw.N;
+  std::vector<int>& ops = c->ops_f[prec][N];
+  if (ops.empty()) ops = adga::lu_ops_factor(N, N);
+  return ops;
 }
 
 // FP64 re-solve of the points `pts` (batch positions) whose complex64 refinement did not converge:
@@ -107,33 +107,35 @@ void ensure_ops(adga_ctx* c, int prec) {
 // factored, solved and checked, and the solutions scattered back into the batch's Y.  Updates the
 // points' entries of relh / infoh and returns the number of matrices re-solved.
 int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>& relh, std::vector<int>& infoh) {
-  const Geom& g = c->g;
+  const Geom& g = c->gw;
   hipStream_t s = c->stream;
   if (pts.empty()) return 0;
-  ensure_ops(c, 0);
+  const std::vector<int>& ops64 = ensure_ops(c, 0);
   FallbackBufs& f = c->fb;
-  const size_t per_mat = (size_t)g.N * c->ldm;
   if (f.cap == 0) {
-    // at most 4 GB of complex128 matrices (two per point), at most 32 points per round
+    // sized for the largest box (c->g); at most 4 GB of complex128 matrices (two per point), at most
+    // 32 points per round
+    const Geom& G = c->g;
+    const size_t per_mat = (size_t)G.N * c->ldm;
     const size_t fit = ((size_t)4 << 30) / (2 * per_mat * sizeof(z64));
     const int cap = (int)std::max<size_t>(1, std::min<size_t>(32, fit));
-    const size_t pmat = (size_t)g.N * g.n2;
-    dalloc(&f.S, (size_t)cap * g.nf * g.n4);
+    const size_t pmat = (size_t)G.N * G.n2;
+    dalloc(&f.S, (size_t)cap * G.nf * G.n4);
     dalloc(&f.B, 2 * cap * pmat);
     dalloc(&f.Y, 2 * cap * pmat);
     dalloc(&f.Z, 2 * cap * pmat);
     dalloc(&f.R, 2 * cap * pmat);
     dalloc(&f.M, 2 * cap * per_mat);
-    dalloc(&f.inv, 2 * cap * adga::lu_inv_elems(g.N));
+    dalloc(&f.inv, 2 * cap * adga::lu_inv_elems(G.N));
     dalloc(&f.rel, 2 * cap);
-    dalloc(&f.piv, 2 * cap * (size_t)g.N);
+    dalloc(&f.piv, 2 * cap * (size_t)G.N);
     dalloc(&f.info, 2 * cap);
     dalloc(&f.sel, cap);
     f.cap = cap;
   }
   if (c->vq && !f.W) {
-    dalloc(&f.W, (size_t)f.cap * g.N * g.n2);
-    dalloc(&f.Q, (size_t)f.cap * g.n4);
+    dalloc(&f.W, (size_t)f.cap * c->g.N * c->g.n2);
+    dalloc(&f.Q, (size_t)f.cap * c->g.n4);
   }
   const long slen = (long)g.nf * g.n4, blen = 2L * g.N * g.n2;
   std::vector<double> r;
@@ -150,7 +152,7 @@ int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>&
       adga::launch_gather(k, (long)g.N * g.n2, f.sel, c->W, f.W, false, s);
       adga::launch_vq_build<double>(g, k, c->ldm, fv, f.M, s);
     }
-    if (!adga::launch_lu<double>(2 * k, g.N, g.N, c->ldm, c->ops_f[0], f.M, f.piv, f.inv, f.info, s) ||
+    if (!adga::launch_lu<double>(2 * k, g.N, g.N, c->ldm, ops64, f.M, f.piv, f.inv, f.info, s) ||
         !adga::launch_lu_solve<double>(2 * k, g.N, g.n2, c->ldm, f.M, f.piv, f.inv, f.B, f.Y, false, s))
       throw HipError{hipErrorInvalidValue, "FP64 re-solve: BSE size outside the LU kernel's range"};
     adga::launch_residual(g, k, f.S, c->cd, c->cm, f.B, f.Y, f.Z, f.R, f.rel, s, c->vq ? &fv : nullptr);
@@ -181,7 +183,7 @@ int fallback_fp64(adga_ctx* c, const std::vector<int>& pts, std::vector<double>&
 // the slowest matrix.
 // infoh must have been queued for download (the complex64 LU's zero-pivot flags) on the stream.
 int refine(adga_ctx* c, int nb, std::vector<double>& relh, std::vector<int>& infoh, std::vector<int>& resolved) {
-  const Geom& g = c->g;
+  const Geom& g = c->gw;
   hipStream_t s = c->stream;
   const int nmat = 2 * nb;
   enum { ACTIVE = 0, DONE = 1, FP64 = 2 };
@@ -245,6 +247,7 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     g.n2 = g.n * g.n;
     g.n4 = g.n2 * g.n2;
     g.iwf = d->iwfmax_small;
+    g.iws = g.iwf;
     g.nf = 2 * g.iwf;
     g.iwbmax = d->iwbmax_small;
     g.N = g.n2 * g.nf;
@@ -258,6 +261,8 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     g.beta = d->beta;
     g.mu = d->mu;
     c->ldm = (g.N + 15) / 16 * 16;
+    c->gw = g;
+    c->eom_g = g;
     c->q0.resize(g.nqp);
     for (int i = 0; i < g.nqp; ++i) {
       if (q_data[i] < 1 || q_data[i] > g.nkp) throw HipError{hipErrorInvalidValue, "q_data out of range"};
@@ -381,15 +386,19 @@ int adga_set_vq(adga_ctx* c, const double* v_q) {
   });
 }
 
-int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* xm, double* chid_out,
-                         double* chim_out, double* bubloc_out) {
+int adga_set_local_omega_box(adga_ctx* c, int iwb, int iwf_box, const double* xd, const double* xm, double* chid_out,
+                             double* chim_out, double* bubloc_out) {
   if (!c || !xd || !xm) return ADGA_E_ARG;
   if (iwb < -c->g.iwbmax || iwb > c->g.iwbmax) return fail(c, ADGA_E_ARG, "iwb outside [-iwbmax_small, iwbmax_small]");
+  if (iwf_box < 1 || iwf_box > c->g.iwf) return fail(c, ADGA_E_ARG, "box outside [1, iwfmax_small]");
   return guarded(c, [&]() -> int {
-    const Geom& g = c->g;
     hipStream_t s = c->stream;
-    if (c->eom_pending && c->eom_iwb != iwb) eom_flush(c);
-    if (c->bub_iwb != iwb) c->bub_valid = false;
+    if (c->eom_pending && (c->eom_iwb != iwb || c->eom_g.iwf != iwf_box)) eom_flush(c);
+    if (c->bub_iwb != iwb || c->bub_iwf != iwf_box) c->bub_valid = false;
+    Geom g = c->g;  // the box of this omega: N = n^2 * 2 * iwf_box
+    g.iwf = iwf_box;
+    g.nf = 2 * iwf_box;
+    g.N = g.n2 * g.nf;
     size_t NN = (size_t)g.N * g.N;
     upload(c->xd, xd, NN, s);
     upload(c->xm, xm, NN, s);
@@ -400,10 +409,17 @@ int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* x
     if (chim_out) CK(hipMemcpyAsync(chim_out, c->chiloc_m, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
     if (bubloc_out) CK(hipMemcpyAsync(bubloc_out, c->bubloc, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
+    c->gw = g;
     c->iwb = iwb;
     c->have_omega = true;
     return ADGA_OK;
   });
+}
+
+int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* xm, double* chid_out,
+                         double* chim_out, double* bubloc_out) {
+  if (!c) return ADGA_E_ARG;
+  return adga_set_local_omega_box(c, iwb, c->g.iwf, xd, xm, chid_out, chim_out, bubloc_out);
 }
 
 int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double* chid, double* chim,
@@ -414,7 +430,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
   for (int p = 0; p < npts; ++p)
     if (iq[p] < 1 || iq[p] > c->g.nqp) return fail(c, ADGA_E_ARG, "iq outside [1, nqp]");
   return guarded(c, [&]() -> int {
-    const Geom& g = c->g;
+    const Geom& g = c->gw;  // the BSE box of the current omega
     hipStream_t s = c->stream;
     const int n2 = g.n2;
     std::fill(c->stats, c->stats + ADGA_NSTATS, 0.0);
@@ -453,11 +469,12 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         adga::launch_eom_fock(g, nb, c->qpos, c->qidx, c->keom, c->vq, c->nfock, c->sig_hf, s);
       }
       CK(hipEventRecord(ev[1], s));
-      const bool use_dft = c->bubble_mode == 1 || (c->bubble_mode == 2 && ((c->bub_valid && c->bub_iwb == c->iwb) || nb >= 64));
+      const bool bub_hit = c->bub_valid && c->bub_iwb == c->iwb && c->bub_iwf == g.iwf;
+      const bool use_dft = c->bubble_mode == 1 || (c->bubble_mode == 2 && (bub_hit || nb >= 64));
       if (use_dft) {
         ensure_conv(c);
-        if (!(c->bub_valid && c->bub_iwb == c->iwb)) {
-          size_t nb_all = (size_t)g.nf * g.n4 * g.nkp;
+        if (!bub_hit) {
+          size_t nb_all = (size_t)c->g.nf * g.n4 * g.nkp;  // sized for the largest box
           if (!c->bub_p) {
             dalloc(&c->bub_p, nb_all);
             dalloc(&c->bub_all, nb_all);
@@ -465,6 +482,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
           adga::launch_bubble_all(g, c->iwb, c->tw, c->ghat, c->bub_p, c->bub_all, s);
           c->bub_valid = true;
           c->bub_iwb = c->iwb;
+          c->bub_iwf = g.iwf;
         }
         adga::launch_bubble_gather(g, nb, c->qidx, c->bub_all, c->chi0q, s);
       } else {
@@ -478,7 +496,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       int iters = 0;
       double maxrel = 0.0;
       bool lu_ok;
-      ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
+      const std::vector<int>& ops = ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
       relh.assign(nmat, 0.0);
       infoh.assign(nmat, 0);
       resolved.assign(nmat, 0);
@@ -487,7 +505,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         if (vtp) adga::launch_vq_build<double>(g, nb, c->ldm, vt, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
-        lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, c->ops_f[0], (z64*)c->M, c->piv,
+        lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, ops, (z64*)c->M, c->piv,
                                         (z64*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
                                         c->ev_lag, c->ev_join);
         CK(hipEventRecord(c->ev_lu[1], s));
@@ -506,7 +524,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         if (vtp) adga::launch_vq_build<float>(g, nb, c->ldm, vt, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
-        lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, c->ops_f[1], (c32*)c->M, c->piv,
+        lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, ops, (c32*)c->M, c->piv,
                                        (c32*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
                                        c->ev_lag, c->ev_join);
         CK(hipEventRecord(c->ev_lu[1], s));
@@ -552,15 +570,16 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
         if (c->eom_mode == 1) {
           ensure_conv(c);
           if (!c->macc) {
-            size_t nm = (size_t)g.iwf * g.n4 * g.nkp;
+            size_t nm = (size_t)c->g.iwf * g.n4 * g.nkp;  // sized for the largest box
             dalloc(&c->macc, nm);
             dalloc(&c->mwork, nm);
             CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, s));
           }
-          if (c->eom_pending && c->eom_iwb != c->iwb) eom_flush(c);
+          if (c->eom_pending && (c->eom_iwb != c->iwb || c->eom_g.iwf != g.iwf)) eom_flush(c);
           adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s);
           c->eom_pending = true;
           c->eom_iwb = c->iwb;
+          c->eom_g = g;
         } else {
           adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, c->gt, c->sig_nl, s);
         }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <exception>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -30,7 +31,9 @@ struct FallbackBufs {
 };
 
 struct adga_ctx {
-  Geom g{};
+  Geom g{};   // the context's geometry (the largest BSE box; the self-energies' box)
+  Geom gw{};  // the current omega's geometry: g with the BSE box of that omega
+  Geom eom_g{};  // geometry of the omega whose EOM is pending
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
   hipEvent_t ev_lag = nullptr, ev_join = nullptr;
@@ -69,7 +72,7 @@ struct adga_ctx {
   int *act = nullptr, *keep = nullptr;  // per-matrix refinement flags, per-point "enters the self-energy" flags
   FallbackBufs fb;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
-  std::vector<int> ops_f[2];  // LU operation lists (host) per precision
+  std::map<int, std::vector<int>> ops_f[2];  // LU operation lists (host) per precision and BSE size
   // accumulated self-energies
   z64 *sig_nl = nullptr, *sig_dmft = nullptr, *sig_hf = nullptr;
   // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega
@@ -78,7 +81,7 @@ struct adga_ctx {
   bool do_eom = true;   // option "do_eom": the reference's do_eom (off for a q path, main.F90:341-347)
   z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
   bool bub_valid = false;
-  int bub_iwb = 0;
+  int bub_iwb = 0, bub_iwf = 0;
   bool eom_pending = false;
   int eom_iwb = 0;
   double stats[ADGA_NSTATS] = {0};  // see adga_get_stats

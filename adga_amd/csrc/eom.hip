@@ -112,21 +112,21 @@ __global__ void __launch_bounds__(256) k_eom(Geom g, int iwb, int npts, const in
     for (int j = 0; j < g.n; ++j)
       for (int kp = 0; kp < g.n; ++kp) acc = cfma(m[(i * g.n + j) + (col0 + kp) * g.n2], gr[j * g.n + kp], acc);
   }
-  long o = i + (long)l * g.n + (long)(g.iwf + nu) * g.n2 + (long)kk * g.n2 * g.nf;
+  long o = i + (long)l * g.n + (long)(g.iws + nu) * g.n2 + (long)kk * g.n2 * (2 * g.iws);
   sigma[o] = cadd(sigma[o], acc);
 }
 
 // Sigma(i, l, -nu-1) = conj(Sigma(l, i, nu)) for nu >= 0 (eom_module.f90:157, 228); nk slices.
 __global__ void k_fill_neg(Geom g, int nk, z64* __restrict__ sigma) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)g.n2 * g.iwf * nk) return;
+  if (idx >= (long)g.n2 * g.iws * nk) return;
   int il = (int)(idx % g.n2);
-  int nu = (int)((idx / g.n2) % g.iwf);
-  long kk = idx / ((long)g.n2 * g.iwf);
+  int nu = (int)((idx / g.n2) % g.iws);
+  long kk = idx / ((long)g.n2 * g.iws);
   int i = il % g.n, l = il / g.n;
-  long base = kk * g.n2 * g.nf;
-  z64 v = sigma[base + l + (long)i * g.n + (long)(g.iwf + nu) * g.n2];
-  sigma[base + i + (long)l * g.n + (long)(g.iwf - nu - 1) * g.n2] = cconj(v);
+  long base = kk * g.n2 * (2 * g.iws);
+  z64 v = sigma[base + l + (long)i * g.n + (long)(g.iws + nu) * g.n2];
+  sigma[base + i + (long)l * g.n + (long)(g.iws - nu - 1) * g.n2] = cconj(v);
 }
 
 // Local EOM with the DMFT g: Sigma_dmft(i, l, nu) += sum_k m(i*n + k, n2*(iwf+nu) + l*n + k) g_k(nu - omega),
@@ -147,7 +147,7 @@ __global__ void k_eom_dmft(Geom g, int iwb, const z64* __restrict__ u, const z64
     z64 gl = giw[(nu - iwb + g.iwmax) + (long)k * 2 * g.iwmax];
     acc = cfma(m, gl, acc);
   }
-  long o = i + (long)l * g.n + (long)(g.iwf + nu) * g.n2;
+  long o = i + (long)l * g.n + (long)(g.iws + nu) * g.n2;
   sdmft[o] = cadd(sdmft[o], acc);
 }
 
@@ -156,7 +156,7 @@ __global__ void k_eom_dmft(Geom g, int iwb, const z64* __restrict__ u, const z64
 __global__ void k_eom_static(Geom g, const z64* __restrict__ u, const z64* __restrict__ ndmft,
                              z64* __restrict__ sdmft) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= g.n * g.nf) return;
+  if (idx >= g.n * 2 * g.iws) return;
   int i = idx % g.n, nu = idx / g.n;
   z64 acc = mk(0.0, 0.0);
   for (int j = 0; j < g.n; ++j) {
@@ -193,15 +193,15 @@ void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const 
 }
 
 void launch_eom_static_local(const Geom& g, const z64* u, const z64* ndmft, z64* sdmft, hipStream_t s) {
-  hipLaunchKernelGGL(k_eom_static, dim3((g.n * g.nf + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
+  hipLaunchKernelGGL(k_eom_static, dim3((g.n * 2 * g.iws + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
 }
 
 void launch_eom_local(const Geom& g, int iwb, bool do_static, const z64* u, const z64* gamd, const z64* giw,
                       const z64* ndmft, z64* sdmft, hipStream_t s) {
   if (do_static)
-    hipLaunchKernelGGL(k_eom_static, dim3((g.n * g.nf + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
+    hipLaunchKernelGGL(k_eom_static, dim3((g.n * 2 * g.iws + 63) / 64), dim3(64), 0, s, g, u, ndmft, sdmft);
   hipLaunchKernelGGL(k_eom_dmft, dim3((g.n2 * g.iwf + 63) / 64), dim3(64), 0, s, g, iwb, u, gamd, giw, sdmft);
-  hipLaunchKernelGGL(k_fill_neg, dim3((g.n2 * g.iwf + 63) / 64), dim3(64), 0, s, g, 1, sdmft);
+  hipLaunchKernelGGL(k_fill_neg, dim3((g.n2 * g.iws + 63) / 64), dim3(64), 0, s, g, 1, sdmft);
 }
 
 __global__ void k_zero_points(int npts, long len, const int* __restrict__ keep, z64* __restrict__ a) {
@@ -234,7 +234,7 @@ void launch_gather(int k, long len, const int* sel, const z64* src, z64* dst, bo
 }
 
 void launch_fill_neg(const Geom& g, int nk, z64* sigma, hipStream_t s) {
-  long tot = (long)g.n2 * g.iwf * nk;
+  long tot = (long)g.n2 * g.iws * nk;
   hipLaunchKernelGGL(k_fill_neg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nk, sigma);
 }
 

@@ -179,7 +179,7 @@ __global__ void k_eom_add(Geom g, const int* __restrict__ keom, const z64* __res
   int i = il % g.n, l = il / g.n;
   int k = keom[kk] - 1;
   z64 v = cscale(hk[((long)nu * g.n2 + i * g.n + l) * g.nkp + k], 1.0 / g.nkp);
-  long o = i + (long)l * g.n + (long)(g.iwf + nu) * g.n2 + kk * g.n2 * g.nf;
+  long o = i + (long)l * g.n + (long)(g.iws + nu) * g.n2 + kk * g.n2 * (2 * g.iws);
   sigma[o] = cadd(sigma[o], v);
 }
 

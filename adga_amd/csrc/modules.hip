@@ -103,16 +103,17 @@ void launch_ndmft(const Geom& g, const z64* giw, z64* nd, hipStream_t s) {
 // sigma_sum(b, b, nu, k) += siw(nu, b) for nu in the box (eom_module.f90:237-251); sigma_sum (n, n, nf, nk).
 __global__ void k_add_siw(Geom g, int nk, const z64* __restrict__ siw, z64* __restrict__ ss) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)g.n * g.nf * nk) return;
+  const int nfs = 2 * g.iws;
+  if (idx >= (long)g.n * nfs * nk) return;
   const int b = (int)(idx % g.n);
-  const int d = (int)((idx / g.n) % g.nf);
-  const long k = idx / ((long)g.n * g.nf);
-  z64* o = ss + b + (long)b * g.n + (long)d * g.n2 + k * g.n2 * g.nf;
-  *o = cadd(*o, siw[(d - g.iwf + g.iwmax) + (long)b * 2 * g.iwmax]);
+  const int d = (int)((idx / g.n) % nfs);
+  const long k = idx / ((long)g.n * nfs);
+  z64* o = ss + b + (long)b * g.n + (long)d * g.n2 + k * g.n2 * nfs;
+  *o = cadd(*o, siw[(d - g.iws + g.iwmax) + (long)b * 2 * g.iwmax]);
 }
 
 void launch_add_siw(const Geom& g, int nk, const z64* siw, z64* ss, hipStream_t s) {
-  const long tot = (long)g.n * g.nf * nk;
+  const long tot = (long)g.n * 2 * g.iws * nk;
   hipLaunchKernelGGL(k_add_siw, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nk, siw, ss);
 }
 

@@ -109,6 +109,19 @@ int adga_set_local_omega(adga_ctx *ctx, int iwb, const double *chi_loc_dens, con
                          double *chi_loc_dens_out, double *chi_loc_magn_out, double *bubble_loc_out);
 
 /*
+ * adga_set_local_omega with a fermionic box of this omega's own: nu in [-iwf_box, iwf_box), 1 <= iwf_box <=
+ * iwfmax_small, so BSE size N(omega) = ndim^2 * 2 * iwf_box and chi_loc_dens/magn(N(omega), N(omega)).  The
+ * following adga_run_points calls solve at that size; the self-energies keep the context's box (the
+ * frequencies outside this omega's box receive nothing from it).  Every quantity of the omega is the
+ * reference's with iwfmax_small = iwf_box.
+ * Replaces: the loop body at one omega with a box that shrinks with |omega| (the reference has one global
+ * iwfmax_small, src/config_module.f90:469).
+ */
+int adga_set_local_omega_box(adga_ctx *ctx, int iwb, int iwf_box, const double *chi_loc_dens,
+                             const double *chi_loc_magn, double *chi_loc_dens_out, double *chi_loc_magn_out,
+                             double *bubble_loc_out);
+
+/*
  * Runs the per-point work for npts q points at the current bosonic frequency: bubble, BSE build,
  * both channel solves, eta, the chi contraction and the dynamic EOM (Sigma_nl accumulated on the
  * device), plus the local EOM for every point with iq == 1 (calc_eom_static at iwb == 0 and
