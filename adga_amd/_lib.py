@@ -18,6 +18,7 @@ ADGA_PREC_FP64, ADGA_PREC_REFINED = 0, 1
 EXPORTS = ["adga_ctx_create", "adga_set_option", "adga_set_local_omega", "adga_run_points", "adga_get_sigma",
            "adga_reset_sigma", "adga_get_stats", "adga_last_error", "adga_ctx_destroy", "adga_inverse_matrix_z",
            "adga_get_giw", "adga_get_nfock", "adga_get_sigma_g_loc", "adga_get_ndga", "adga_set_vq", "adga_set_local_omega_box",
+           "adga_set_tangents", "adga_set_local_omega_tangent", "adga_run_points_tangent", "adga_get_sigma_tangent",
            "adga_bubble_q", "adga_chi0_loc", "adga_chi_qw", "adga_eom_dynamic", "adga_eom_dmft", "adga_eom_static",
            "adga_get_gkiw", "adga_get_ndmft", "adga_add_siw_dmft",
            "adga_inverse_matrix_z_noalloc"]
@@ -42,6 +43,10 @@ def _load():
         "adga_ctx_create": ([C.POINTER(adga_dims), _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)], C.c_int),
         "adga_set_option": ([_vp, C.c_char_p, C.c_double], C.c_int),
         "adga_set_vq": ([_vp, _dp], C.c_int),
+        "adga_set_tangents": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "adga_set_local_omega_tangent": ([_vp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "adga_run_points_tangent": ([_vp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip], C.c_int),
+        "adga_get_sigma_tangent": ([_vp, _dp], C.c_int),
         "adga_bubble_q": ([_vp, C.c_int, C.c_int, _dp], C.c_int),
         "adga_chi0_loc": ([_vp, C.c_int, _dp, _dp], C.c_int),
         "adga_chi_qw": ([_vp, _dp, _dp, _dp], C.c_int),

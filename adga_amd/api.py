@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import ADGA_OK, ADGA_PREC_FP64, ADGA_PREC_REFINED, lib
 
-__all__ = ["AdgaError", "Context", "run_loop", "inverse_matrix", "PRECISION"]
+__all__ = ["AdgaError", "Context", "run_loop", "run_loop_tangent", "inverse_matrix", "PRECISION"]
 
 PRECISION = {"fp64": ADGA_PREC_FP64, "refined": ADGA_PREC_REFINED}
 
@@ -193,6 +193,51 @@ class Context:
         self._check(lib.adga_get_ndga(self.h, _dptr(ss), _dptr(ndk), _dptr(nd)))
         return ndk, nd
 
+    # ----------------------------------------------------------------- derivatives (tangent-linear mode)
+    def set_tangents(self, tan):
+        """Directions of the inputs to differentiate along: a synth.Tangents (or any object with dhk, dsiw,
+        dgiw, dmu, du, du_tilde in those shapes, the last axis the direction)."""
+        self.ndir = tan.dhk.shape[-1]
+        arrs = [_zf(tan.dhk), _zf(tan.dsiw), _zf(tan.dgiw)]
+        dmu = np.ascontiguousarray(tan.dmu, dtype=np.float64)
+        du, dut = _zf(tan.du), _zf(tan.du_tilde)
+        self._check(lib.adga_set_tangents(self.h, self.ndir, *[_dptr(a) for a in arrs], _dptr(dmu), _dptr(du),
+                                          _dptr(dut)))
+
+    def set_local_omega_tangent(self, dchi_loc_dens=None, dchi_loc_magn=None):
+        """After set_local_omega: derivatives of its local data given those of chi_loc (N, N, ndir) or None;
+        returns the derivatives of (chi_loc_d, chi_loc_m, bubble_loc), each (n2, n2, ndir)."""
+        n2, nd = self.n2, self.ndir
+        xd = None if dchi_loc_dens is None else _zf(dchi_loc_dens)
+        xm = None if dchi_loc_magn is None else _zf(dchi_loc_magn)
+        od = np.zeros((n2, n2, nd), dtype=np.complex128, order="F")
+        om, bl = np.zeros_like(od), np.zeros_like(od)
+        self._check(lib.adga_set_local_omega_tangent(self.h, None if xd is None else _dptr(xd),
+                                                     None if xm is None else _dptr(xm), _dptr(od), _dptr(om),
+                                                     _dptr(bl)))
+        return od, om, bl
+
+    def run_points_tangent(self, iq):
+        """run_points in complex128 with the derivatives dchi_d, dchi_m, dbubble_nl (n2, n2, npts, ndir)."""
+        iq = np.ascontiguousarray(iq, dtype=np.int32)
+        npts, n2, nd = iq.size, self.n2, self.ndir
+        out = {k: np.zeros((n2, n2, npts), dtype=np.complex128, order="F") for k in ("chi_d", "chi_m", "bubble_nl")}
+        for k in ("dchi_d", "dchi_m", "dbubble_nl"):
+            out[k] = np.zeros((n2, n2, npts, nd), dtype=np.complex128, order="F")
+        st = np.zeros(npts, dtype=np.int32)
+        rc = lib.adga_run_points_tangent(self.h, npts, iq.ctypes.data_as(_lib._ip), _dptr(out["chi_d"]),
+                                         _dptr(out["chi_m"]), _dptr(out["bubble_nl"]), _dptr(out["dchi_d"]),
+                                         _dptr(out["dchi_m"]), _dptr(out["dbubble_nl"]), st.ctypes.data_as(_lib._ip))
+        self._check(rc, st)
+        out["status"] = st
+        return out
+
+    def get_sigma_tangent(self):
+        n = self.n
+        d = np.zeros((n, n, self.nf, self.nkp_eom, self.ndir), dtype=np.complex128, order="F")
+        self._check(lib.adga_get_sigma_tangent(self.h, _dptr(d)))
+        return d
+
     def reset_sigma(self):
         self._check(lib.adga_reset_sigma(self.h))
 
@@ -238,6 +283,41 @@ def run_loop(ctx: Context, points, chi_loc_fn, precision="fp64", box_fn=None):
         out["status"][p:e] = r["status"]
         p = e
     out["sigma_nl"], out["sigma_dmft"], out["sigma_hf"] = ctx.get_sigma()
+    return out
+
+
+def run_loop_tangent(ctx: Context, points, chi_loc_fn, dchi_loc_fn, box_fn=None):
+    """run_loop in complex128 with the derivatives along the context's tangent directions
+    (Context.set_tangents): `dchi_loc_fn(iwb, channel)` gives d chi_loc (N, N, ndir) of a channel.  Returns
+    run_loop's quantities and d<name> for each of them (last axis: the direction)."""
+    n2, nd = ctx.n2, ctx.ndir
+    npts = len(points)
+    out = {k: np.zeros((n2, n2, npts), dtype=np.complex128, order="F") for k in ("chi_d", "chi_m", "bubble_nl")}
+    for k in ("dchi_d", "dchi_m", "dbubble_nl"):
+        out[k] = np.zeros((n2, n2, npts, nd), dtype=np.complex128, order="F")
+    out["status"] = np.zeros(npts, dtype=np.int32)
+    for k in ("chi_loc_d", "chi_loc_m", "bubble_loc", "dchi_loc_d", "dchi_loc_m", "dbubble_loc"):
+        out[k] = {}
+    p = 0
+    while p < npts:
+        iwb = points[p][0]
+        e = p
+        while e < npts and points[e][0] == iwb:
+            e += 1
+        box = None if box_fn is None else box_fn(iwb)
+        ld, lm, bl = ctx.set_local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"), box)
+        dld, dlm, dbl = ctx.set_local_omega_tangent(dchi_loc_fn(iwb, "dens"), dchi_loc_fn(iwb, "magn"))
+        if points[p][1] == 0:
+            for k, v in (("chi_loc_d", ld), ("chi_loc_m", lm), ("bubble_loc", bl), ("dchi_loc_d", dld),
+                         ("dchi_loc_m", dlm), ("dbubble_loc", dbl)):
+                out[k][iwb] = v
+        r = ctx.run_points_tangent(np.array([q + 1 for (_, q) in points[p:e]], dtype=np.int32))
+        for k in ("chi_d", "chi_m", "bubble_nl", "dchi_d", "dchi_m", "dbubble_nl"):
+            out[k][:, :, p:e] = r[k]
+        out["status"][p:e] = r["status"]
+        p = e
+    out["sigma_nl"], out["sigma_dmft"], out["sigma_hf"] = ctx.get_sigma()
+    out["dsigma_nl"] = ctx.get_sigma_tangent()
     return out
 
 

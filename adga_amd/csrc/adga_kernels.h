@@ -18,12 +18,20 @@ bool launch_op_ndga(const Geom& g, const z64* hk, const z64* siw, const z64* dc,
 void launch_local(const Geom& g, int iwb, const z64* giw, const z64* xd, const z64* xm, z64* winv, z64* chi0w,
                   z64* cd, z64* cm, z64* gam_d, z64* gam_m, z64* opg_d, z64* opg_m, z64* chiloc_d,
                   z64* chiloc_m, z64* bubloc, hipStream_t s);
-void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s);
+// gt2 (may be null: gt) is the table of G(K - q, nu - omega); accumulate adds into chi0q
+void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s,
+                   const z64* gt2 = nullptr, bool accumulate = false);
+// Out = C_ch^T Z_ch for the channels ch0 .. ch0 + nch - 1 (C row-major as the left operand, Z and Out in the
+// residual layouts), plus B - Y when B is given, plus Out itself when accumulate
+void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* Z, z64* Out,
+                const z64* B, const z64* Y, hipStream_t s, bool accumulate = false);
 void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64* S, hipStream_t s);
 template <typename T>
 void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd, const z64* cm, cplx<T>* M,
                   hipStream_t s);
 void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s);
+// the right operand of gamma^q_d's product (S of each point in the residual product's layout)
+void launch_gamq_z(const Geom& g, int npts, const z64* S, z64* Z, hipStream_t s);
 size_t lu_inv_elems(int N);
 std::vector<int> lu_ops_factor(int N, int W);
 // the same list written into buf (at most cap ints); returns the number of ints the list needs
@@ -95,9 +103,26 @@ void launch_ndmft(const Geom& g, const z64* giw, z64* nd, hipStream_t s);
 void launch_add_siw(const Geom& g, int nk, const z64* siw, z64* ss, hipStream_t s);
 // with vq non-null: V(q_b) = vq + qpos[b] n4 enters the density eta term and -V gamma^w_d / (beta nqp) is added
 void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
-                 hipStream_t s, const int* qpos = nullptr, const z64* vq = nullptr, const z64* gamd = nullptr);
+                 hipStream_t s, const int* qpos = nullptr, const z64* vq = nullptr, const z64* gamd = nullptr,
+                 bool accumulate = false);
 void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,
                        const z64* gt, z64* sigma, hipStream_t s);
+// tangent-linear kernels (tangent.hip)
+bool launch_dgtable(const Geom& g, int ndir, const z64* gt, const z64* dhk, const z64* dsiw, const double* dmu,
+                    z64* dgt, hipStream_t s);
+void launch_local_tangent(const Geom& g, int iwb, const z64* giw, const z64* dgiw, const z64* xd, const z64* xm,
+                          const z64* dxd, const z64* dxm, const z64* winv, const z64* gamd, const z64* gamm,
+                          const z64* chi0w, z64* dw, z64* dwinv, z64* dcd, z64* dcm, z64* dgamd, z64* dgamm,
+                          z64* dchiloc_d, z64* dchiloc_m, z64* dbubloc, hipStream_t s);
+void launch_dsmat(const Geom& g, int npts, const z64* chi0q, const z64* dchi0q, const z64* winv, const z64* dwinv,
+                  z64* dS, hipStream_t s);
+void launch_tangent_rhs(const Geom& g, int npts, const z64* S, const z64* dS, const z64* cd, const z64* cm,
+                        const z64* dcd, const z64* dcm, const z64* opgd, const z64* opgm, const z64* dopgd,
+                        const z64* dopgm, const z64* Y, z64* Z1, z64* Z2, z64* B, hipStream_t s);
+void launch_dchi(const Geom& g, int npts, const z64* chi0q, const z64* dchi0q, const z64* chi0w, const z64* dchi0w,
+                 const z64* gamd, const z64* gamm, const z64* dgamd, const z64* dgamm, const z64* Y, const z64* dY,
+                 z64* dchi_out, z64* dbub_out, hipStream_t s);
+void launch_eta(const Geom& g, int nmat, const z64* Y, z64* eta, hipStream_t s);
 // k-grid convolutions (kconv.hip)
 void dft3(const Geom& g, long nfield, int sign, const z64* tw, z64* a, z64* b, hipStream_t s);
 void launch_ghat(const Geom& g, const z64* tw, const z64* gt, z64* tmp, z64* ghat, hipStream_t s);

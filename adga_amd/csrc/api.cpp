@@ -422,9 +422,13 @@ int adga_set_local_omega(adga_ctx* c, int iwb, const double* xd, const double* x
   return adga_set_local_omega_box(c, iwb, c->g.iwf, xd, xm, chid_out, chim_out, bubloc_out);
 }
 
-int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double* chid, double* chim,
-                    double* bubnl, int* status) {
-  if (!c || npts < 0 || (npts > 0 && !iq)) return ADGA_E_ARG;
+}  // extern "C"
+
+namespace {
+// the per-point work of adga_run_points; with `tan` (adga_run_points_tangent) also the derivatives of every
+// batch along the context's tangent directions
+int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double* chid, double* chim, double* bubnl,
+                    int* status, const TangentOut* tan) {
   if (precision != ADGA_PREC_FP64 && precision != ADGA_PREC_REFINED) return fail(c, ADGA_E_ARG, "bad precision");
   if (!c->have_omega) return fail(c, ADGA_E_STATE, "adga_set_local_omega must be called before adga_run_points");
   for (int p = 0; p < npts; ++p)
@@ -586,6 +590,7 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
       }
       CK(hipEventRecord(ev[6], s));
       CK(hipGetLastError());
+      if (tan) tangent_batch(c, nb, p0, npts, keep, nfailed, *tan);
       chih.resize((size_t)2 * nb * g.n4);
       bubh.resize((size_t)nb * g.n4);
       CK(hipMemcpyAsync(chih.data(), c->chi_out, sizeof(z64) * chih.size(), hipMemcpyDeviceToHost, s));
@@ -624,6 +629,25 @@ int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double*
     else if (worst == ADGA_E_NONFINITE) c->err = "non-finite value in a BSE solve";
     return worst;
   });
+}
+}  // namespace
+
+extern "C" {
+
+int adga_run_points(adga_ctx* c, int npts, const int* iq, int precision, double* chid, double* chim,
+                    double* bubnl, int* status) {
+  return run_points_impl(c, npts, iq, precision, chid, chim, bubnl, status, nullptr);
+}
+
+int adga_run_points_tangent(adga_ctx* c, int npts, const int* iq, double* chid, double* chim, double* bubnl,
+                            double* dchid, double* dchim, double* dbubnl, int* status) {
+  if (!c) return ADGA_E_ARG;
+  if (c->ndir == 0) return fail(c, ADGA_E_STATE, "adga_set_tangents must be called before adga_run_points_tangent");
+  if (!c->t_omega || c->t_iwb != c->iwb || c->t_iwf != c->gw.iwf)
+    return fail(c, ADGA_E_STATE, "adga_set_local_omega_tangent must follow the omega's adga_set_local_omega");
+  if (c->vq) return fail(c, ADGA_E_ARG, "the derivatives are formed without V(q)");
+  const TangentOut tan{dchid, dchim, dbubnl};
+  return run_points_impl(c, npts, iq, ADGA_PREC_FP64, chid, chim, bubnl, status, &tan);
 }
 
 int adga_get_sigma(adga_ctx* c, double* sig_nl, double* sig_dmft, double* sig_hf) {
@@ -719,6 +743,7 @@ int adga_reset_sigma(adga_ctx* c) {
     CK(hipMemsetAsync(c->sig_nl, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, c->stream));
     CK(hipMemsetAsync(c->sig_dmft, 0, sizeof(z64) * g.n2 * g.nf, c->stream));
     CK(hipMemsetAsync(c->sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom, c->stream));
+    if (c->t_sig) CK(hipMemsetAsync(c->t_sig, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom * c->ndir, c->stream));
     if (c->macc) CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp, c->stream));
     c->eom_pending = false;
     c->bub_valid = false;  // the next batch forms the bubble of its omega again
@@ -750,6 +775,7 @@ int adga_ctx_destroy(adga_ctx* c) {
   dfree(c->sig_nl); dfree(c->sig_dmft); dfree(c->sig_hf);
   dfree(c->vq); dfree(c->nfock); dfree(c->W); dfree(c->vqQ); dfree(c->qpos);
   for (z64*& p : c->stage) dfree(p);
+  free_tangent(c);
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
   for (auto e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -20,9 +20,12 @@ namespace adga {
 // chi0q[b][d](i1 = i*n + j, i2 = l*n + k') = -beta/nkp sum_K G(K, nu_d)_il G(K - q_b, nu_d - omega)_k'j
 // (accumulate_chi0, one_particle_quant_module.f90:274-288; k-sum main.F90:647-657).
 // One block per (nu, point); thread = ((i,l), k-slice); n^2 accumulators per thread.
+// gt2 is the table of the second factor G(K - q, nu - omega) (gt itself, or a derivative table when the
+// tangent of the bubble is formed by the product rule); accumulate adds into chi0q.
 template <int NN>
 __global__ void __launch_bounds__(256) k_bubble(Geom g, int iwb, const int* __restrict__ qidx,
-                                                const z64* __restrict__ gt, z64* __restrict__ chi0q) {
+                                                const z64* __restrict__ gt, const z64* __restrict__ gt2,
+                                                z64* __restrict__ chi0q, int accumulate) {
   constexpr int N2 = NN * NN;
   constexpr int NKS = 256 / N2;
   __shared__ z64 red[256];
@@ -32,7 +35,7 @@ __global__ void __launch_bounds__(256) k_bubble(Geom g, int iwb, const int* __re
   const int q = qidx[b];
   const int iwf = d - g.iwf;
   const z64* g1 = gt + (long)(iwf - g.glo) * g.nkp * N2;
-  const z64* g2 = gt + (long)(iwf - iwb - g.glo) * g.nkp * N2;
+  const z64* g2 = gt2 + (long)(iwf - iwb - g.glo) * g.nkp * N2;
   z64 acc[N2];
 #pragma unroll
   for (int e = 0; e < N2; ++e) acc[e] = mk(0.0, 0.0);
@@ -56,20 +59,24 @@ __global__ void __launch_bounds__(256) k_bubble(Geom g, int iwb, const int* __re
       for (int kk = 0; kk < NKS; ++kk) s = cadd(s, red[kk * N2 + t]);
       // thread t = og = (i, l); e = (k', j): chi0(i1 = i*n + j, i2 = l*n + k')
       int ii = t / NN, ll = t % NN, kp = e / NN, jj = e % NN;
-      out[(ii * NN + jj) + (ll * NN + kp) * N2] = cscale(s, f);
+      z64* o = out + (ii * NN + jj) + (ll * NN + kp) * N2;
+      *o = accumulate ? cadd(*o, cscale(s, f)) : cscale(s, f);
     }
     __syncthreads();
   }
 }
 
-void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s) {
+void launch_bubble(const Geom& g, int iwb, int npts, const int* qidx, const z64* gt, z64* chi0q, hipStream_t s,
+                   const z64* gt2, bool accumulate) {
   dim3 grid(g.nf, npts);
+  const z64* g2 = gt2 ? gt2 : gt;
+  const int acc = accumulate ? 1 : 0;
   switch (g.n) {
-    case 1: hipLaunchKernelGGL(k_bubble<1>, grid, dim3(256), 0, s, g, iwb, qidx, gt, chi0q); break;
-    case 2: hipLaunchKernelGGL(k_bubble<2>, grid, dim3(256), 0, s, g, iwb, qidx, gt, chi0q); break;
-    case 3: hipLaunchKernelGGL(k_bubble<3>, grid, dim3(256), 0, s, g, iwb, qidx, gt, chi0q); break;
-    case 4: hipLaunchKernelGGL(k_bubble<4>, grid, dim3(256), 0, s, g, iwb, qidx, gt, chi0q); break;
-    case 5: hipLaunchKernelGGL(k_bubble<5>, grid, dim3(256), 0, s, g, iwb, qidx, gt, chi0q); break;
+    case 1: hipLaunchKernelGGL(k_bubble<1>, grid, dim3(256), 0, s, g, iwb, qidx, gt, g2, chi0q, acc); break;
+    case 2: hipLaunchKernelGGL(k_bubble<2>, grid, dim3(256), 0, s, g, iwb, qidx, gt, g2, chi0q, acc); break;
+    case 3: hipLaunchKernelGGL(k_bubble<3>, grid, dim3(256), 0, s, g, iwb, qidx, gt, g2, chi0q, acc); break;
+    case 4: hipLaunchKernelGGL(k_bubble<4>, grid, dim3(256), 0, s, g, iwb, qidx, gt, g2, chi0q, acc); break;
+    case 5: hipLaunchKernelGGL(k_bubble<5>, grid, dim3(256), 0, s, g, iwb, qidx, gt, g2, chi0q, acc); break;
     default: break;
   }
 }
@@ -263,8 +270,8 @@ __global__ void k_resid_z(Geom g, int npts, const z64* __restrict__ S, const z64
 constexpr int RG_TM = 128, RG_TN = 64, RG_WR = 4, RG_WC = 2, RG_KC = 16;
 __global__ void __launch_bounds__(RG_WR * RG_WC * 64)
 k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* __restrict__ cm,
-             const z64* __restrict__ Z, z64* __restrict__ Out, const z64* __restrict__ B,
-             const z64* __restrict__ Y) {
+             const z64* __restrict__ Z, z64* Out, const z64* __restrict__ B,
+             const z64* __restrict__ Y, int accumulate) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[G3Cfg<double, RG_TM, RG_TN, RG_WR, RG_WC, RG_KC>::LDS];
   const int ch = ch0 + (int)blockIdx.y;
   const z64* C = ch == 0 ? cd : cm;
@@ -282,17 +289,19 @@ k_resid_gemm(Geom g, int npts, int ch0, const z64* __restrict__ cd, const z64* _
       C, N, Z + (long)ch * N * ncol, ncol,
       [=](int r, int col) {
         const long i = at(r, col);
-        return csub(B[i], Y[i]);
+        return B ? csub(B[i], Y[i]) : Out[i];
       },
-      [=](int r, int col, z64 v) { Out[at(r, col)] = v; }, N, ncol, N, B == nullptr, false, rt, ct, lds);
+      [=](int r, int col, z64 v) { Out[at(r, col)] = v; }, N, ncol, N, B == nullptr && !accumulate, false, rt, ct,
+      lds);
 }
 
-static void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* Z,
-                       z64* Out, const z64* B, const z64* Y, hipStream_t s) {
+// Out = C_ch^T Z_ch (+ B - Y when B is given, + Out itself when accumulate)
+void resid_gemm(const Geom& g, int npts, int ch0, int nch, const z64* cd, const z64* cm, const z64* Z, z64* Out,
+                const z64* B, const z64* Y, hipStream_t s, bool accumulate) {
   const int ntr = (g.N + RG_TM - 1) / RG_TM, ntc = (npts * g.n2 + RG_TN - 1) / RG_TN;
   const int nwg = ntr * ((ntc + 7) / 8) * 8;
   hipLaunchKernelGGL(k_resid_gemm, dim3(nwg, nch), dim3(RG_WR * RG_WC * 64), 0, s, g, npts, ch0, cd, cm, Z, Out, B,
-                     Y);
+                     Y, accumulate ? 1 : 0);
 }
 
 // Per-matrix max |R| / max |B| (|.| = |re|+|im|), one workgroup per matrix; a zero right-hand
@@ -333,10 +342,14 @@ __global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __res
   Z[idx] = S[((long)b * g.nf + d) * g.n4 + s * g.n2 + ip];
 }
 
-// gamma^q_d for every point of the batch, into G2 with the residual layout (point b at 2*b).
-void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
+void launch_gamq_z(const Geom& g, int npts, const z64* S, z64* Z, hipStream_t s) {
   long tot = (long)npts * g.N * g.n2;
   hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
+}
+
+// gamma^q_d for every point of the batch, into G2 with the residual layout (point b at 2*b).
+void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z64* G2, hipStream_t s) {
+  launch_gamq_z(g, npts, S, Z, s);
   resid_gemm(g, npts, 0, 1, cd, cd, Z, G2, nullptr, nullptr, s);
 }
 

@@ -85,10 +85,28 @@ struct adga_ctx {
   bool eom_pending = false;
   int eom_iwb = 0;
   double stats[ADGA_NSTATS] = {0};  // see adga_get_stats
+  // tangent directions (adga_set_tangents): derivative of the G table, of giw, mu, U and U~ per direction
+  int ndir = 0;
+  z64 *dgt = nullptr, *dgiw = nullptr, *du = nullptr, *dut = nullptr;
+  double* dmu = nullptr;
+  // derivatives of the current omega's local data, per direction (adga_set_local_omega_tangent)
+  bool t_omega = false;
+  int t_iwb = 0, t_iwf = 0;
+  z64 *t_dw = nullptr, *t_dwinv = nullptr, *t_dcd = nullptr, *t_dcm = nullptr, *t_dgamd = nullptr, *t_dgamm = nullptr;
+  // per-batch derivative workspace and the derivative of Sigma_nl per direction
+  int t_cap = 0;
+  z64 *t_dchi0q = nullptr, *t_dS = nullptr, *t_Z1 = nullptr, *t_Z2 = nullptr, *t_B = nullptr, *t_dY = nullptr,
+      *t_deta = nullptr, *t_dgamq = nullptr, *t_dmt = nullptr, *t_dchi = nullptr, *t_dbub = nullptr;
+  z64* t_sig = nullptr;
   // device staging of caller arrays for the module-level entry points (api_modules.cpp)
   z64* stage[6] = {};
   size_t stage_n[6] = {};
   adga::LuTimer lut;       // events around the dominant GEMM kernel's launches
+};
+
+// derivative outputs of adga_run_points_tangent: (ndim^2, ndim^2, npts, ndir) each, may be NULL
+struct TangentOut {
+  double *dchid, *dchim, *dbub;
 };
 
 namespace adga_host {
@@ -168,5 +186,11 @@ inline int op_finish(adga_ctx* c, bool launched, const char* what) {
   CK(hipGetLastError());
   return ADGA_OK;
 }
+
+// derivatives of one batch of points (positions p0 .. p0 + nb - 1 of npts) along every tangent direction,
+// after the forward work of the batch (api_tangent.cpp)
+void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>& keep, int nfailed,
+                   const TangentOut& out);
+void free_tangent(adga_ctx* c);
 
 }  // namespace adga_host

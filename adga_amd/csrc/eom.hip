@@ -54,7 +54,8 @@ __global__ void k_chi(Geom g, int npts, const z64* __restrict__ chi0q, const z64
 // Output (n2, N) Fortran per point.
 __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* __restrict__ ut,
                        const z64* __restrict__ eta, const z64* __restrict__ gamq, z64* __restrict__ mt,
-                       const int* __restrict__ qpos, const z64* __restrict__ vq, const z64* __restrict__ gamd) {
+                       const int* __restrict__ qpos, const z64* __restrict__ vq, const z64* __restrict__ gamd,
+                       int accumulate) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long per = (long)g.n2 * g.N;
   if (idx >= per * npts) return;
@@ -84,7 +85,7 @@ __global__ void k_mtot(Geom g, int npts, const z64* __restrict__ u, const z64* _
   m = cadd(cscale(t2, a2), m);
   m = cadd(cscale(t3, a3), m);
   if (v) m = cadd(cscale(t4, a1), m);
-  mt[idx] = m;
+  mt[idx] = accumulate ? cadd(mt[idx], m) : m;
 }
 
 // Sigma_nl(i, l, nu, k) += sum_b sum_{j,k'} m_b(i*n + j, n2*(iwf + nu) + l*n + k') G(k - q_b, nu - omega)_{jk'}
@@ -179,10 +180,10 @@ void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, cons
 }
 
 void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
-                 hipStream_t s, const int* qpos, const z64* vq, const z64* gamd) {
+                 hipStream_t s, const int* qpos, const z64* vq, const z64* gamd, bool accumulate) {
   long tot = (long)npts * g.n2 * g.N;
   hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt,
-                     qpos, vq, gamd);
+                     qpos, vq, gamd, accumulate ? 1 : 0);
 }
 
 void launch_eom_direct(const Geom& g, int iwb, int npts, const int* qidx, const int* keom, const z64* mt,

@@ -30,7 +30,7 @@ import math
 import numpy as np
 
 __all__ = ["Workload", "Inputs", "WORKLOADS", "workload", "build_inputs", "chi_loc",
-           "kanamori_u", "generate_q_vol", "k_minus_q"]
+           "kanamori_u", "generate_q_vol", "k_minus_q", "PARAMS", "Tangents", "tangents", "dchi_loc"]
 
 
 @dataclasses.dataclass(frozen=True)
@@ -314,29 +314,123 @@ def k_minus_q(ik, iq, nk):
             + ((nkx + ix - lx) % nkx) * nky * nkz)
 
 
+def _hubbard_i(wl):
+    """Parameters (h, s, e) of the orbital-diagonal Hubbard-I-like self-energy h + s^2 / (i nu - e); each is
+    proportional to U."""
+    n = wl.n
+    rng = np.random.default_rng([wl.seed, 1])
+    if n == 1:
+        return np.array([wl.U / 2]), np.array([wl.U / 2]), np.zeros(1)
+    h = np.full(n, wl.U / 2) + 0.02 * wl.U * rng.normal(size=n)
+    s = wl.U / 2 * (1.0 + 0.05 * rng.normal(size=n))
+    e = 0.05 * wl.U * rng.normal(size=n)
+    return h, s, e
+
+
+def _g_blocks(hk, siw, dc, mu, nu, iwmax, block=64):
+    """G(k, nu) for nu >= 0 in blocks of frequencies: yields (ws, G[w, k, a, b])."""
+    n = hk.shape[0]
+    hkt = np.ascontiguousarray(hk.transpose(2, 0, 1))  # (nkp, n, n)
+    eye = np.eye(n)
+    for i0 in range(0, iwmax, block):
+        ws = np.arange(i0, min(iwmax, i0 + block))
+        a = -hkt[None, :, :, :] + np.zeros((ws.size, 1, 1, 1))
+        diag = (1j * nu[ws + iwmax][:, None] + mu - dc[0][None, :] - siw[ws + iwmax])  # (nw, n)
+        a = a + diag[:, None, :, None] * eye[None, None, :, :]
+        yield ws, np.linalg.inv(a)
+
+
+def _hk(wl):
+    if wl.model == "sc1":
+        return _hk_sc1(wl)
+    if wl.model == "t2g":
+        return _hk_t2g(wl)
+    return _hk_rand(wl)
+
+
+PARAMS = ("t", "tp", "U")
+
+
+@dataclasses.dataclass
+class Tangents:
+    """Derivatives of the hidden inputs with respect to the model parameters PARAMS = (t, t', U), the
+    directions of the GPU path's tangent-linear mode: dhk(n, n, nkp, 3), dsiw and dgiw (2*iwmax, n, 3),
+    dmu (3,), du and du_tilde (n^2, n^2, 3).  dc and the chi_loc noise do not depend on them."""
+    dhk: np.ndarray
+    dsiw: np.ndarray
+    dgiw: np.ndarray
+    dmu: np.ndarray
+    du: np.ndarray
+    du_tilde: np.ndarray
+
+
+def tangents(inp: Inputs) -> Tangents:
+    """d(input)/d(t, t', U) of build_inputs: H(k) is linear in t and t' (t2g, sc1) or proportional to t
+    (random hopping); h, s, e of the self-energy and mu = U/2 are proportional to U; U and U~ are linear in
+    U at fixed J; dG = G (dH + dSigma - dmu) G gives dgiw = 1/nkp sum_k diag dG."""
+    wl = inp.wl
+    n, iwmax = wl.n, inp.iwmax
+    nkp = wl.nkp
+    dhk = np.zeros((n, n, nkp, 3), dtype=np.complex128)
+    if wl.model == "rand5":
+        dhk[..., 0] = inp.hk / wl.t if wl.t != 0 else 0.0
+    else:
+        dhk[..., 0] = _hk(dataclasses.replace(wl, t=1.0, tp=0.0))
+        if wl.model == "t2g":
+            dhk[..., 1] = _hk(dataclasses.replace(wl, t=0.0, tp=1.0))
+    h, s, e = _hubbard_i(wl)
+    nu = inp.iw_data
+    den = 1j * nu[:, None] - e[None, :]
+    dsiw = np.zeros((2 * iwmax, n, 3), dtype=np.complex128)
+    dsiw[..., 2] = (h[None, :] + 2 * s[None, :] ** 2 / den + s[None, :] ** 2 * e[None, :] / den ** 2) / wl.U
+    dmu = np.array([0.0, 0.0, 0.5])
+    dgiw = np.zeros((2 * iwmax, n, 3), dtype=np.complex128)
+    hkt = np.ascontiguousarray(dhk.transpose(2, 3, 0, 1))  # (nkp, 3, n, n)
+    for ws, g in _g_blocks(inp.hk, inp.siw, inp.dc, inp.mu, nu, iwmax):
+        for d in range(3):
+            x = np.broadcast_to(hkt[None, :, d], g.shape).copy()
+            x[..., np.arange(n), np.arange(n)] += (dsiw[ws + iwmax, :, d] - dmu[d])[:, None, :]
+            dg = g @ x @ g
+            dgiw[ws + iwmax, :, d] = np.einsum("wkaa->wa", dg) / nkp
+    for w in range(iwmax):
+        dgiw[-w - 1 + iwmax] = np.conj(dgiw[w + iwmax])
+    du = np.zeros((n * n, n * n, 3), dtype=np.complex128)
+    dut = np.zeros_like(du)
+    du[..., 2], dut[..., 2] = kanamori_u(n, 1.0, 0.0)
+    return Tangents(np.asfortranarray(dhk), np.asfortranarray(dsiw), np.asfortranarray(dgiw), dmu,
+                    np.asfortranarray(du), np.asfortranarray(dut))
+
+
+def dchi_loc(inp: Inputs, tan: Tangents, iwb: int, box: int | None = None) -> np.ndarray:
+    """d chi_loc_r(omega) / d(t, t', U), (N, N, 3), the same for both channels: the local bubble's
+    -beta d(g_i(nu) g_j(nu - omega)) on the diagonal (the noise is fixed)."""
+    wl = inp.wl
+    n = wl.n
+    iwf = wl.iwfmax_small if box is None else box
+    nf = 2 * iwf
+    N = n * n * nf
+    off = inp.iwmax
+    out = np.zeros((N, N, 3), dtype=np.complex128, order="F")
+    for d in range(nf):
+        w = d - iwf
+        gi, gj = inp.giw[w + off], inp.giw[w - iwb + off]
+        for p in range(3):
+            dgi, dgj = tan.dgiw[w + off, :, p], tan.dgiw[w - iwb + off, :, p]
+            v = -inp.beta * (dgi[:, None] * gj[None, :] + gi[:, None] * dgj[None, :])
+            idx = d * n * n + np.arange(n * n)
+            out[idx, idx, p] = v.reshape(-1)
+    return out
+
+
 def build_inputs(wl: Workload) -> Inputs:
     n = wl.n
     beta = wl.beta
     iwmax = wl.n_iwmax
     if iwmax < wl.iwfmax_small + wl.iwbmax_small:
         raise ValueError("iwmax must be >= iwfmax_small + iwbmax_small (config_module.f90:600-607)")
-    if wl.model == "sc1":
-        hk = _hk_sc1(wl)
-    elif wl.model == "t2g":
-        hk = _hk_t2g(wl)
-    else:
-        hk = _hk_rand(wl)
+    hk = _hk(wl)
     assert hk.shape[0] == n
-    rng = np.random.default_rng([wl.seed, 1])
-    # Hubbard-I-like orbital-diagonal self-energy
-    if n == 1:
-        h = np.array([wl.U / 2])
-        s = np.array([wl.U / 2])
-        e = np.zeros(1)
-    else:
-        h = np.full(n, wl.U / 2) + 0.02 * wl.U * rng.normal(size=n)
-        s = wl.U / 2 * (1.0 + 0.05 * rng.normal(size=n))
-        e = 0.05 * wl.U * rng.normal(size=n)
+    h, s, e = _hubbard_i(wl)
     mu = float(wl.U / 2)
     iw = np.arange(-iwmax, iwmax)
     nu = math.pi * (2 * iw + 1) / beta
@@ -344,14 +438,7 @@ def build_inputs(wl: Workload) -> Inputs:
     dc = np.zeros((2, n), dtype=np.complex128)
     # g(nu) = 1/nkp sum_k diag G(k, nu) for nu >= 0, conjugate for nu < 0 (get_giw)
     giw = np.zeros((2 * iwmax, n), dtype=np.complex128)
-    hkt = np.ascontiguousarray(hk.transpose(2, 0, 1))  # (nkp, n, n)
-    eye = np.eye(n)
-    for i0 in range(0, iwmax, 64):
-        ws = np.arange(i0, min(iwmax, i0 + 64))
-        a = -hkt[None, :, :, :] + np.zeros((ws.size, 1, 1, 1))
-        diag = (1j * nu[ws + iwmax][:, None] + mu - dc[0][None, :] - siw[ws + iwmax])  # (nw, n)
-        a = a + diag[:, None, :, None] * eye[None, None, :, :]
-        g = np.linalg.inv(a)
+    for ws, g in _g_blocks(hk, siw, dc, mu, nu, iwmax):
         giw[ws + iwmax] = np.einsum("wkaa->wa", g) / hk.shape[2]
     for w in range(iwmax):
         giw[-w - 1 + iwmax] = np.conj(giw[w + iwmax])

@@ -222,6 +222,38 @@ int adga_get_ndmft(adga_ctx *ctx, double *n_dmft);
  */
 int adga_add_siw_dmft(adga_ctx *ctx, double *sigma_sum);
 
+/*
+ * Derivatives (tangent-linear form) of the per-point outputs and of Sigma_nl along ndir directions of the
+ * inputs, for gradients with respect to model parameters; complex128 throughout, no V(q).  Per direction:
+ *   dhk(ndim,ndim,nkp,ndir), dsiw/dgiw(2*iwmax,ndim,ndir), dmu(ndir) (real), du/du_tilde(ndim^2,ndim^2,ndir)
+ * (dc is held fixed).  The derivative of the G table is formed at once (dG = G (dH + dSigma - dmu) G).
+ * No counterpart in the reference (it has no gradients).
+ */
+int adga_set_tangents(adga_ctx *ctx, int ndir, const double *dhk, const double *dsiw, const double *dgiw,
+                      const double *dmu, const double *du, const double *du_tilde);
+
+/*
+ * After adga_set_local_omega(_box) at an omega: the derivatives of its local data, given the derivatives
+ * of chi_loc_dens/magn (N(omega), N(omega), ndir; NULL: zero).  Outputs (may be NULL), (ndim^2, ndim^2, ndir):
+ * the derivatives of chi_loc_dens/magn_out and bubble_loc_out of adga_set_local_omega.
+ */
+int adga_set_local_omega_tangent(adga_ctx *ctx, const double *dchi_loc_dens, const double *dchi_loc_magn,
+                                 double *dchi_loc_dens_out, double *dchi_loc_magn_out, double *dbubble_loc_out);
+
+/*
+ * adga_run_points in complex128 that also returns, per direction, the derivatives of chi_dens, chi_magn and
+ * bubble_nl (ndim^2, ndim^2, npts, ndir; may be NULL) and accumulates the derivative of Sigma_nl on the
+ * device.  Each direction reuses the batch's LU factors: a right-hand side and a solve, the bubble formed
+ * again with one factor differentiated at a time, and the EOM twice.
+ */
+int adga_run_points_tangent(adga_ctx *ctx, int npts, const int *iq, double *chi_dens_out, double *chi_magn_out,
+                            double *bubble_nl_out, double *dchi_dens_out, double *dchi_magn_out,
+                            double *dbubble_nl_out, int *status);
+
+/* dsigma_nl(ndim, ndim, 2*iwfmax_small, nkp_eom, ndir): the accumulated derivative of sigma_nl (zeroed by
+   adga_set_tangents and adga_reset_sigma). */
+int adga_get_sigma_tangent(adga_ctx *ctx, double *dsigma_nl);
+
 /* Zeroes the accumulated self-energies and drops the per-omega bubble cache (the next batch forms the
    bubble of its omega again). */
 int adga_reset_sigma(adga_ctx *ctx);

@@ -1,0 +1,87 @@
+"""Derivatives of chi_m(q, omega = 0) and Sigma_DGA(k, nu) with respect to (t, t', U) on the GPU (the
+tangent-linear mode of the hot path) and the torch.autograd.Function around them (adga_amd/grad.py).
+
+* Against the FP64 reference gradient: four-point central differences (step 1e-3) of the numpy oracle on the
+  3-orbital 4^3 workload, every derivative array compared (a sum over a subset of the (omega, q) points, three
+  bosonic frequencies).
+* At the configuration of the differentiation workload (3-orbital t2g, 12^3 k, 2x40 fermionic x 21 bosonic,
+  N = 720, every point): torch's gradient of a scalar loss against four-point central differences of the GPU
+  forward pass.
+The derivatives are small next to the values (d Sigma / dt ~ 5e-5 with |Sigma| ~ 14), so a two-point difference
+at a step small enough for its h^2 error carries a round-off of eps |Sigma| / (h |dSigma|) ~ 1e-6; the
+four-point stencil [8 (f(h) - f(-h)) - (f(2h) - f(-2h))] / 12h at h = 1e-3 has an h^4 error below 1e-8 and a
+round-off below 1e-7 here, so the bound below (1e-6 relative) sits well above both and far below the error of
+a missing term."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from adga_amd import synth
+
+pytestmark = pytest.mark.gpu
+H = 1e-3
+
+
+def _four_point(f, theta, p):
+    """[8 (f(+h) - f(-h)) - (f(+2h) - f(-2h))] / 12h along parameter p; f returns a tuple of arrays."""
+    def at(s):
+        e = np.zeros(3)
+        e[p] = s * H
+        return f(theta + e)
+    f1, g1, f2, g2 = at(1), at(-1), at(2), at(-2)
+    return tuple((8 * (a - b) - (c - d)) / (12 * H) for a, b, c, d in zip(f1, g1, f2, g2))
+
+
+def _oracle_outputs(wl, points):
+    import adga_oracle
+    inp = synth.build_inputs(wl)
+    out = adga_oracle.HotPath(inp).run(points, lambda w, c: synth.chi_loc(inp, w, c))
+    hp = adga_oracle.HotPath(inp)
+    fin = hp.finalize(out, points)
+    sel = [p for p, (w, _) in enumerate(points) if w == 0]
+    return fin["chi_m"][:, :, sel], fin["sigma_dga"]
+
+
+def test_gpu_tangents_match_oracle_differences():
+    from adga_amd import grad
+    wl = synth.workload("t2g_k4")
+    points = [(w, q) for w in (-1, 0, 1) for q in range(0, 64, 4)]
+    print("GPU tangent pass, t2g_k4", flush=True)
+    dchi, dsig = grad.tangent_arrays(wl, points)
+    theta = np.array([wl.t, wl.tp, wl.U])
+    for p, name in enumerate(synth.PARAMS):
+        print(f"oracle central difference along {name}", flush=True)
+        fd_chi, fd_sig = _four_point(lambda th: _oracle_outputs(grad.with_params(wl, th), points), theta, p)
+        e_chi, e_sig = rel_err(dchi[..., p], fd_chi), rel_err(dsig[..., p], fd_sig)
+        print(f"  d/d{name}: chi {e_chi:.2e}, sigma {e_sig:.2e}", flush=True)
+        assert e_chi < 1e-6 and e_sig < 1e-6, name
+
+
+@pytest.mark.timeout(600)
+def test_gpu_autograd_t2g_k12_against_central_differences():
+    """About a minute: 14 passes over the 36288 points (forward, tangent, 12 for the differences)."""
+    import torch
+    from adga_amd import grad
+    wl = synth.workload("t2g_k12")
+    theta = torch.tensor([wl.t, wl.tp, wl.U], dtype=torch.float64, requires_grad=True)
+    print("forward t2g_k12 (36288 points, N = 720)", flush=True)
+    chi, sig = grad.ChiSigma.apply(theta, wl)
+    rng = np.random.default_rng(5)
+    wc = torch.from_numpy(rng.normal(size=chi.shape) + 1j * rng.normal(size=chi.shape))
+    ws = torch.from_numpy(rng.normal(size=sig.shape) + 1j * rng.normal(size=sig.shape))
+
+    def loss(c, s):
+        return (torch.conj(wc) * c).real.sum() + (torch.conj(ws) * s).real.sum() / s.numel() * c.numel()
+
+    L = loss(chi, sig)
+    print("backward (tangent pass)", flush=True)
+    L.backward()
+    g = theta.grad.numpy()
+    th = theta.detach().numpy()
+    for p, name in enumerate(synth.PARAMS):
+        print(f"central difference along {name}", flush=True)
+        fd, = _four_point(lambda t: (np.array(loss(*[torch.from_numpy(a) for a in
+                                                      grad.forward_arrays(grad.with_params(wl, t))]).item()),),
+                          th, p)
+        print(f"  dL/d{name}: autograd {g[p]:.12e}, central difference {fd:.12e}", flush=True)
+        assert abs(g[p] - fd) <= 1e-6 * max(abs(fd), 1e-3 * abs(L.item())), name
