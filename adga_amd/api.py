@@ -45,7 +45,7 @@ def _dptr(a):
 
 class Context:
     def __init__(self, inp, batch=None, refine_tol=None, refine_maxit=None, bubble=None, eom=None, lu_streams=None,
-                 do_eom=None, fallback=None):
+                 do_eom=None, fallback=None, bubble_range=None):
         wl = inp.wl
         self.n = wl.n
         self.n2 = wl.n * wl.n
@@ -73,9 +73,11 @@ class Context:
             self.set_vq(inp.v_q)
         if do_eom is None:
             do_eom = getattr(wl, "do_eom", True)  # the reference runs a q path without the EOM
+        if bubble_range is None:
+            bubble_range = getattr(wl, "bubble_range", 0) or None  # the debug option "Bubble" (+ the tail)
         for key, v in (("batch", batch), ("refine_tol", refine_tol), ("refine_maxit", refine_maxit),
                        ("bubble", bubble), ("eom", eom), ("lu_streams", lu_streams), ("do_eom", int(bool(do_eom))),
-                       ("fallback", fallback)):
+                       ("fallback", fallback), ("bubble_range", bubble_range)):
             if v is not None:
                 self._check(lib.adga_set_option(self.h, key.encode(), float(v)))
 

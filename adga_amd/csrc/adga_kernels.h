@@ -131,5 +131,11 @@ void launch_bubble_gather(const Geom& g, int npts, const int* qidx, const z64* a
 void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s);
 void launch_eom_flush(const Geom& g, int iwb, const int* keom, const z64* tw, const z64* ghat, z64* macc, z64* work,
                       z64* sigma, hipStream_t s);
+// bubble beyond the box: large range and Matsubara tail (tail.hip)
+void launch_bubloc_range(const Geom& g, int iwb, int lo, int hi, const z64* giw, z64* bubloc, hipStream_t s);
+void launch_tail(const Geom& g, int iwb, int K, const z64* giw, const z64* bubloc, z64* out, hipStream_t s);
+void launch_bubble_outer(const Geom& g, int iwb, int lo, int hi, const z64* tw, const z64* ghat, z64* P, z64* out,
+                         hipStream_t s);
+void launch_bubble_outer_add(const Geom& g, int npts, const int* qidx, const z64* all, z64* bub_out, hipStream_t s);
 
 }  // namespace adga

@@ -86,6 +86,28 @@ void ensure_conv(adga_ctx* c) {
   dfree(tmp);
 }
 
+void eom_flush(adga_ctx* c);
+
+// the G table and its k-grid transform over every frequency of the 1P data, nu in [-iwmax, iwmax), for the
+// large bubble range (option "bubble_range")
+void widen_gtable(adga_ctx* c) {
+  eom_flush(c);  // a pending EOM reads the transform of the current table
+  const int iwmax = c->g.iwmax;
+  for (Geom* g : {&c->g, &c->gw, &c->eom_g}) {
+    g->glo = -iwmax;
+    g->nw = 2 * iwmax;
+  }
+  dfree(c->gt);
+  dalloc(&c->gt, (size_t)c->g.nw * c->g.nkp * c->g.n2);
+  adga::launch_gtable(c->g, c->hk, c->siw, c->dc, c->gt, c->stream);
+  CK(hipGetLastError());
+  dfree(c->ghat);
+  dfree(c->tw);  // both formed again by ensure_conv
+  c->bub_valid = false;
+  c->outer_valid = false;
+  CK(hipStreamSynchronize(c->stream));
+}
+
 void eom_flush(adga_ctx* c) {
   if (!c->eom_pending) return;
   adga::launch_eom_flush(c->eom_g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream);
@@ -347,6 +369,17 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
     c->bubble_mode = (int)v;
     return ADGA_OK;
   }
+  if (!strcmp(key, "bubble_range")) {
+    if (!(v == 0 || v == 1 || v == 2))
+      return fail(c, ADGA_E_ARG, "bubble_range must be 0 (box), 1 (large range) or 2 (large range and tail)");
+    if (v >= 1 && c->ndir > 0) return fail(c, ADGA_E_STATE, "bubble_range 1 or 2 has no tangent-linear form");
+    return guarded(c, [&]() -> int {
+      if (v >= 1 && c->g.glo != -c->g.iwmax) widen_gtable(c);
+      if ((int)v != c->bubble_range) c->have_omega = false;  // bubble_loc of the current omega is stale
+      c->bubble_range = (int)v;
+      return ADGA_OK;
+    });
+  }
   if (!strcmp(key, "fallback")) {
     if (!(v == 0 || v == 1)) return fail(c, ADGA_E_ARG, "fallback must be 0 or 1");
     c->fallback = v == 1;
@@ -395,6 +428,7 @@ int adga_set_local_omega_box(adga_ctx* c, int iwb, int iwf_box, const double* xd
     hipStream_t s = c->stream;
     if (c->eom_pending && (c->eom_iwb != iwb || c->eom_g.iwf != iwf_box)) eom_flush(c);
     if (c->bub_iwb != iwb || c->bub_iwf != iwf_box) c->bub_valid = false;
+    if (c->outer_iwb != iwb || c->outer_iwf != iwf_box) c->outer_valid = false;
     Geom g = c->g;  // the box of this omega: N = n^2 * 2 * iwf_box
     g.iwf = iwf_box;
     g.nf = 2 * iwf_box;
@@ -404,10 +438,20 @@ int adga_set_local_omega_box(adga_ctx* c, int iwb, int iwf_box, const double* xd
     upload(c->xm, xm, NN, s);
     adga::launch_local(g, iwb, c->giw, c->xd, c->xm, c->winv, c->chi0w, c->cd, c->cm, c->gamd, c->gamm, c->opgd,
                        c->opgm, c->chiloc_d, c->chiloc_m, c->bubloc, s);
+    // bubble_loc over the large range [-K, K-1] (main.F90:513-520 with the debug option "Bubble"); it is
+    // also what bubble_nl subtracts, while the tail enters the returned bubble_loc only
+    const int K = c->g.iwmax - c->g.iwbmax;
+    if (c->bubble_range >= 1) adga::launch_bubloc_range(g, iwb, -K, K - 1, c->giw, c->bubloc, s);
+    const z64* bl = c->bubloc;
+    if (c->bubble_range == 2) {
+      if (!c->bubout) dalloc(&c->bubout, (size_t)g.n4);
+      adga::launch_tail(g, iwb, K, c->giw, c->bubloc, c->bubout, s);
+      bl = c->bubout;
+    }
     CK(hipGetLastError());
     if (chid_out) CK(hipMemcpyAsync(chid_out, c->chiloc_d, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
     if (chim_out) CK(hipMemcpyAsync(chim_out, c->chiloc_m, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
-    if (bubloc_out) CK(hipMemcpyAsync(bubloc_out, c->bubloc, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
+    if (bubloc_out) CK(hipMemcpyAsync(bubloc_out, bl, sizeof(z64) * g.n4, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     c->gw = g;
     c->iwb = iwb;
@@ -564,6 +608,23 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       CK(hipEventRecord(ev[4], s));
       adga::launch_eta_chi(g, nb, c->Y, c->opgd, c->opgm, c->eta, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc,
                            c->chi_out, c->bub_out, s);
+      if (c->bubble_range >= 1) {
+        // the large range's frequencies outside the box enter bubble_nl (main.F90:644-663, debug option
+        // "Bubble"): all q of the omega as one convolution, kept for the omega's next batches
+        ensure_conv(c);
+        if (!(c->outer_valid && c->outer_iwb == c->iwb && c->outer_iwf == g.iwf)) {
+          if (!c->outer_p) {
+            dalloc(&c->outer_p, (size_t)g.n4 * g.nkp);
+            dalloc(&c->outer_all, (size_t)g.n4 * g.nkp);
+          }
+          const int K = c->g.iwmax - c->g.iwbmax;
+          adga::launch_bubble_outer(g, c->iwb, -K, K - 1, c->tw, c->ghat, c->outer_p, c->outer_all, s);
+          c->outer_valid = true;
+          c->outer_iwb = c->iwb;
+          c->outer_iwf = g.iwf;
+        }
+        adga::launch_bubble_outer_add(g, nb, c->qidx, c->outer_all, c->bub_out, s);
+      }
       CK(hipEventRecord(ev[5], s));
       if (c->do_eom) {
         adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s, c->qpos, c->vq, c->gamd);
@@ -646,6 +707,7 @@ int adga_run_points_tangent(adga_ctx* c, int npts, const int* iq, double* chid, 
   if (!c->t_omega || c->t_iwb != c->iwb || c->t_iwf != c->gw.iwf)
     return fail(c, ADGA_E_STATE, "adga_set_local_omega_tangent must follow the omega's adga_set_local_omega");
   if (c->vq) return fail(c, ADGA_E_ARG, "the derivatives are formed without V(q)");
+  if (c->bubble_range != 0) return fail(c, ADGA_E_STATE, "the derivatives are formed with the bubble on the box");
   const TangentOut tan{dchid, dchim, dbubnl};
   return run_points_impl(c, npts, iq, ADGA_PREC_FP64, chid, chim, bubnl, status, &tan);
 }
@@ -777,6 +839,7 @@ int adga_ctx_destroy(adga_ctx* c) {
   for (z64*& p : c->stage) dfree(p);
   free_tangent(c);
   dfree(c->tw); dfree(c->ghat); dfree(c->bub_p); dfree(c->bub_all); dfree(c->macc); dfree(c->mwork);
+  dfree(c->outer_p); dfree(c->outer_all); dfree(c->bubout);
   for (auto e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto e : c->ev_lu)

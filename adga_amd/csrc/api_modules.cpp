@@ -35,7 +35,7 @@ int adga_bubble_q(adga_ctx* c, int iwb, int q_k, double* chi0q) {
   if (!iwb_ok(c, iwb)) return fail(c, ADGA_E_ARG, "iwb outside [-iwbmax_small, iwbmax_small]");
   if (q_k < 1 || q_k > c->g.nkp) return fail(c, ADGA_E_ARG, "q_k outside [1, nkp]");
   return guarded(c, [&]() -> int {
-    const Geom& g = c->g;
+    const Geom g = bubble_geom(c);
     hipStream_t s = c->stream;
     const size_t ne = (size_t)g.nf * g.n4;
     int* qd = reinterpret_cast<int*>(stage(c, 0, 1));
@@ -54,7 +54,7 @@ int adga_chi0_loc(adga_ctx* c, int iwb, double* chi0w_diag, double* chi0w_inv_di
   if (!c || (!chi0w_diag && !chi0w_inv_diag)) return ADGA_E_ARG;
   if (!iwb_ok(c, iwb)) return fail(c, ADGA_E_ARG, "iwb outside [-iwbmax_small, iwbmax_small]");
   return guarded(c, [&]() -> int {
-    const Geom& g = c->g;
+    const Geom g = bubble_geom(c);
     hipStream_t s = c->stream;
     const size_t ne = (size_t)g.n2 * g.nf;
     z64* w = stage(c, 0, ne);

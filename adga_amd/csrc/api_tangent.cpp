@@ -120,6 +120,7 @@ int adga_set_tangents(adga_ctx* c, int ndir, const double* dhk, const double* ds
                       const double* dmu, const double* du, const double* dut) {
   if (!c || ndir < 1 || ndir > 16 || !dhk || !dsiw || !dgiw || !dmu || !du || !dut) return ADGA_E_ARG;
   if (c->eom_pending) return fail(c, ADGA_E_STATE, "tangents cannot change while an omega is pending (adga_get_sigma)");
+  if (c->bubble_range != 0) return fail(c, ADGA_E_STATE, "the derivatives are formed with the bubble on the box");
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     hipStream_t s = c->stream;

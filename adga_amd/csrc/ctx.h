@@ -82,6 +82,12 @@ struct adga_ctx {
   z64 *tw = nullptr, *ghat = nullptr, *bub_p = nullptr, *bub_all = nullptr, *macc = nullptr, *mwork = nullptr;
   bool bub_valid = false;
   int bub_iwb = 0, bub_iwf = 0;
+  // option "bubble_range": 0 the box, 1 every frequency the 1P data allow (the reference's debug option
+  // "Bubble"; the G table then covers [-iwmax, iwmax)), 2 that and the Matsubara tail in bubble_loc
+  int bubble_range = 0;
+  z64 *outer_p = nullptr, *outer_all = nullptr, *bubout = nullptr;  // outer-frequency bubble of all q, one omega
+  bool outer_valid = false;
+  int outer_iwb = 0, outer_iwf = 0;
   bool eom_pending = false;
   int eom_iwb = 0;
   double stats[ADGA_NSTATS] = {0};  // see adga_get_stats
@@ -176,6 +182,19 @@ inline void ensure_nfock(adga_ctx* c) {
   dalloc(&c->nfock, (size_t)c->g.nkp * c->g.n2);
   if (!adga::launch_op_nfock(c->g, c->hk, c->siw, c->dc, c->nfock, c->stream))
     throw HipError{hipErrorInvalidValue, "n_fock: orbital count above 8"};
+}
+
+// the context's geometry with the "box" set to the bubble's frequency range: the vertex box, or with the
+// option bubble_range >= 1 every frequency for which G(nu - omega) exists, [-iwmax + iwbmax, iwmax - iwbmax)
+// (iwstart/iwstop of the debug option "Bubble", config_module.f90:467-477)
+inline Geom bubble_geom(const adga_ctx* c) {
+  Geom g = c->g;
+  if (c->bubble_range >= 1) {
+    g.iwf = g.iwmax - g.iwbmax;
+    g.nf = 2 * g.iwf;
+    g.N = g.n2 * g.nf;
+  }
+  return g;
 }
 
 inline int op_finish(adga_ctx* c, bool launched, const char* what) {

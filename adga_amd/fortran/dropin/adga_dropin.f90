@@ -35,8 +35,11 @@ contains
     endif
     call dropin_release()
     if (ndim > 5) call mpi_stop('adga: the GPU modules take at most 5 orbitals', 1)
-    if (iwstart /= -iwfmax_small .or. iwstop /= iwfmax_small-1) &
-      call mpi_stop('adga: the GPU modules take the bubble on the vertex box only (no debug option Bubble)', 1)
+    ! the bubble's frequencies: the vertex box, or with the debug option "Bubble" the large range
+    ! (config_module.f90:467-477), which the library's option bubble_range = 1 selects
+    if ((iwstart /= -iwfmax_small .or. iwstop /= iwfmax_small-1) .and. &
+        (iwstart /= -iwmax+iwbmax_small .or. iwstop /= iwmax-iwbmax_small-1)) &
+      call mpi_stop('adga: iwstart/iwstop must be the vertex box or the range of the debug option Bubble', 1)
     d%ndim = ndim; d%nkpx = nkpx; d%nkpy = nkpy; d%nkpz = nkpz; d%iwmax = iwmax
     d%iwfmax_small = iwfmax_small; d%iwbmax_small = iwbmax_small
     d%beta = beta; d%mu = mu
@@ -60,6 +63,10 @@ contains
     d%nqp = size(ql); d%nkp_eom = size(kl)
     rc = c_adga_ctx_create(d, ql, kl, hk, siw, gi, dc, uu, ut, nd, ctx)
     if (rc /= ADGA_OK) call mpi_stop('adga: GPU context creation failed (adga_ctx_create)', int(rc))
+    if (iwstart /= -iwfmax_small) then
+      rc = c_adga_set_option(ctx, 'bubble_range'//c_null_char, 1d0)
+      if (rc /= ADGA_OK) call mpi_stop('adga: adga_set_option(bubble_range) failed', int(rc))
+    endif
     ctx_full = full
     c = ctx
   end function dropin_ctx

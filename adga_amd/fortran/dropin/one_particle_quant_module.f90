@@ -61,8 +61,7 @@ subroutine local_bubble(iwb)
   integer, intent(in) :: iwb
   integer(c_int) :: rc
   if (iwb == loc_w) return
-  if (.not. allocated(loc)) allocate(loc(ndim2, -iwfmax_small:iwfmax_small-1), &
-                                     loc_inv(ndim2, -iwfmax_small:iwfmax_small-1))
+  if (.not. allocated(loc)) allocate(loc(ndim2, iwstart:iwstop), loc_inv(ndim2, iwstart:iwstop))
   rc = c_adga_chi0_loc(dropin_ctx(.true.), int(iwb, c_int), loc, loc_inv)
   call dropin_check(rc, 'get_chi0_loc')
   loc_w = iwb
@@ -102,7 +101,7 @@ subroutine accumulate_chi0(ik, ikq, iwf, iwb, chi0)
   kx = (ikq - 1)/(nkpz*nkpy)
   qk = 1 + mod(nkpz - kz, nkpz) + mod(nkpy - ky, nkpy)*nkpz + mod(nkpx - kx, nkpx)*nkpy*nkpz
   if (qk /= bub_q .or. iwb /= bub_w) then
-    if (.not. allocated(bub)) allocate(bub(ndim2, ndim2, -iwfmax_small:iwfmax_small-1))
+    if (.not. allocated(bub)) allocate(bub(ndim2, ndim2, iwstart:iwstop))
     rc = c_adga_bubble_q(dropin_ctx(.true.), int(iwb, c_int), int(qk, c_int), bub)
     call dropin_check(rc, 'accumulate_chi0')
     bub_q = qk

@@ -53,6 +53,9 @@ class Workload:
     k_eom: str = "all"          # EOM k list: "all" (the grid in order) or "sub" (every third k, reversed)
     q_list: str = "vol"         # q list: "vol" (generate_q_vol on qgrid) or "path" (a path through the grid)
     vq: float = 0.0             # scale of the synthetic non-local interaction V(q) (0: no V(q))
+    bubble_range: int = 0       # bubble sums over the BSE box (0), over every frequency the 1P data
+                                # allow (1, the reference's debug option "Bubble"), or 1 + the
+                                # asymptotic Matsubara tail beyond them (2)
 
     @property
     def nkp(self):
@@ -113,6 +116,11 @@ WORKLOADS = {
                           t=0.3, seed=18, vq=0.4),
     "t2g_k4_vq": Workload("t2g_k4_vq", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
                           t=-0.26, tp=-0.08, seed=19, vq=0.5, k_eom="sub"),
+    # bubble_loc and bubble_nl over the large frequency range (debug option "Bubble")
+    "t2g_k4_bub": Workload("t2g_k4_bub", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
+                           t=-0.26, tp=-0.08, seed=20, iwmax=20, bubble_range=1),
+    "two_k4_bub": Workload("two_k4_bub", 2, (4, 4, 2), 4, 2, 8.0, 3.0, 0.5, "rand5",
+                           t=0.3, seed=21, iwmax=15, bubble_range=1, k_eom="sub"),
     "t2g_k4": Workload("t2g_k4", 3, (4, 4, 4), 6, 2, 10.0, 5.0, 0.75, "t2g",
                        t=-0.26, tp=-0.08, seed=11),
     "t2g_k6": Workload("t2g_k6", 3, (6, 6, 6), 12, 3, 10.0, 5.0, 0.75, "t2g",

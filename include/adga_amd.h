@@ -80,7 +80,15 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
  * G by k-grid DFT when omega changes or the self-energy is read, the default), "lu_streams" (1: each
  * batch's LU on one stream; 2: the batch split in two halves on two streams, the second one
- * operation behind, the large Schur updates on the whole batch; the default; results are identical).
+ * operation behind, the large Schur updates on the whole batch; the default; results are identical),
+ * "bubble_range" (0: bubble_loc and bubble_nl sum chi0 over the vertex box, the default; 1: over every
+ * nu for which G(nu - omega) exists, [-iwmax + iwbmax_small, iwmax - iwbmax_small), the reference's
+ * debug option "Bubble" (src/config_module.f90:467-477); 2: as 1, and the bubble_loc returned by
+ * adga_set_local_omega also holds the Matsubara tail beyond that range, -1/beta sum g_i(nu) g_j(nu-omega)
+ * with g expanded to third order in 1/(i nu), its moments fitted to the two highest frequencies of giw,
+ * in FP64 on the device (no counterpart in the reference; bubble_nl is unchanged by it, its tail being
+ * O(nu^-4)); 1 and 2 widen the device G table to all 2*iwmax frequencies and need adga_set_local_omega
+ * again before the next points; not with tangent directions).
  * Returns ADGA_E_ARG for an unknown key or value.
  */
 int adga_set_option(adga_ctx *ctx, const char *key, double value);
@@ -158,7 +166,9 @@ int adga_get_sigma(adga_ctx *ctx, double *sigma_nl, double *sigma_dmft, double *
 /*
  * chi0q(ndim^2, ndim^2, 2*iwfmax_small): the bubble of the q whose k-grid index is q_k (1-based) at
  * omega iwb, every nu of the box: -beta/nkp sum_k G(k, nu)_il G(k-q, nu-omega)_kj at
- * [(i-1)*ndim + j, (l-1)*ndim + k], i.e. the sum of accumulate_chi0 over ik divided by nkp.
+ * [(i-1)*ndim + j, (l-1)*ndim + k], i.e. the sum of accumulate_chi0 over ik divided by nkp.  With the
+ * option bubble_range >= 1 every nu of the large range instead: 2*(iwmax - iwbmax_small) frequencies
+ * from -iwmax + iwbmax_small (iwstart .. iwstop of the debug option "Bubble").
  * Replaces: accumulate_chi0 over the k loop (src/one_particle_quant_module.f90:228-290,
  * src/main.F90:647-653).
  */
@@ -166,7 +176,8 @@ int adga_bubble_q(adga_ctx *ctx, int iwb, int q_k, double *chi0q);
 
 /*
  * Diagonals (ndim^2, 2*iwfmax_small) of chi0_loc(nu) = -beta giw(nu, i) giw(nu - omega, j) at
- * i1 = (i-1)*ndim + j and of its inverse, for every nu of the box (either pointer may be NULL).
+ * i1 = (i-1)*ndim + j and of its inverse, for every nu of the box (either pointer may be NULL); with
+ * bubble_range >= 1 for every nu of the large range, as adga_bubble_q.
  * Replaces: get_chi0_loc, get_chi0_loc_inv (src/one_particle_quant_module.f90:192-225).
  */
 int adga_chi0_loc(adga_ctx *ctx, int iwb, double *chi0w_diag, double *chi0w_inv_diag);

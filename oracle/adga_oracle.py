@@ -24,6 +24,11 @@ shapes and index meaning (first index fastest, 0-based here):
 * chi assembly              main.F90:930-936, 1047-1063, 1124-1140
 * one-particle routines     one_particle_quant_module.f90:40-79 (get_giw), 124-180
                             (get_sigma_g_loc), 337-361 (get_nfock), 363-393 (get_ndga)
+* large-range bubble        config_module.f90:467-477 (debug option "Bubble": iwstart/iwstop),
+                            main.F90:513-520, 644-663
+* Matsubara tail            no reference counterpart: the local bubble summed beyond the large
+                            range with the asymptotic expansion of giw (`tail_sums`,
+                            `bubble_loc_tail`)
 
 Parity pins (tests/test_oracle_*.py): golden vectors written by the reference's
 own Fortran modules (oracle/_ref, built from the reference sources with
@@ -33,13 +38,16 @@ loop restatement on tiny grids, numpy inverse for the adjugate, the Matsubara
 symmetries; `finalize` (the chi and Sigma assembly after the loop, which the
 reference does inline in main.F90 and no golden vector covers) by the local
 limit (chi = beta^-2 sum_{nu nu'} chi_loc) and the F = 0 limit (chi = the full
-bubble, Sigma_DGA = siw).
+bubble, Sigma_DGA = siw); the Matsubara-tail sums by high-precision summation
+(mpmath) and the tail-corrected local bubble of a flat band by its closed form
+(the Lindhard function [f(a) - f(b)] / (a - b - i omega)).
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["HotPath", "inverse_small", "kmq_table", "get_giw", "get_nfock", "get_sigma_g_loc", "get_ndga"]
+__all__ = ["HotPath", "inverse_small", "kmq_table", "get_giw", "get_nfock", "get_sigma_g_loc", "get_ndga",
+           "tail_sums", "giw_moments", "bubble_loc_tail"]
 
 
 def inverse_small(a):
@@ -164,6 +172,65 @@ def get_ndga(inp, sigma_sum):
     return ndk, nd
 
 
+def tail_sums(beta, K, m):
+    """S[(p, r)] = sum over n outside [-K, K-1] of x^-p y^-r with x = i nu_n, y = i nu_{n-m} = x - i omega_m,
+    for (p, r) in (1,1), (2,1), (1,2), (3,1), (2,2), (1,3): by partial fractions in w = x - y into sums of
+    single powers, X_s = sum x^-s and Y_s = sum y^-s, whose even powers are Hurwitz zeta tails and whose
+    odd powers cancel pairwise (nu_{-n-1} = -nu_n) except for 2|m| terms of the shifted set (the first
+    power as the limit of symmetric truncations, the sums with p + r >= 2 converging absolutely)."""
+    from scipy.special import zeta
+    nu = lambda j: np.pi * (2 * np.asarray(j, dtype=float) + 1) / beta  # noqa: E731
+    c = beta / (2 * np.pi)
+    B = {2: 2 * c ** 2 * zeta(2, K + 0.5), 4: 2 * c ** 4 * zeta(4, K + 0.5)}   # sum nu^-s over the outside
+    X = {1: 0.0, 2: (1j) ** -2 * B[2], 3: 0.0}
+    if m == 0:
+        return {(1, 1): X[2], (2, 1): 0.0, (1, 2): 0.0, (3, 1): B[4], (2, 2): B[4], (1, 3): B[4]}
+    M = abs(m)
+    inner = np.arange(K - M, K + M)                    # the 2|m| unpaired terms of an odd power
+    lower, upper = np.arange(K - M, K), np.arange(K, K + M)
+    A = {1: np.sign(m) * np.sum(1 / nu(inner)), 3: np.sign(m) * np.sum(1 / nu(inner) ** 3),
+         2: B[2] + np.sum(1 / nu(lower) ** 2) - np.sum(1 / nu(upper) ** 2)}
+    Y = {s: (1j) ** -s * A[s] for s in (1, 2, 3)}
+    w = 2j * np.pi * m / beta
+    D = Y[1] - X[1]
+    return {(1, 1): D / w,
+            (2, 1): D / w ** 2 - X[2] / w,
+            (1, 2): Y[2] / w - D / w ** 2,
+            (3, 1): D / w ** 3 - X[2] / w ** 2 - X[3] / w,
+            (2, 2): (Y[2] + X[2]) / w ** 2 - 2 * D / w ** 3,
+            (1, 3): Y[3] / w - Y[2] / w ** 2 + D / w ** 3}
+
+
+def giw_moments(giw, beta, iwmax):
+    """(c1, c2) of giw = 1/(i nu) + c1/(i nu)^2 + c2/(i nu)^3 + c3/(i nu)^4 + ..., per orbital, from the two
+    highest positive frequencies: u = nu^2 Re g = -c1 + c3/nu^2 and v = nu^3 (Im g + 1/nu) = c2 - c4/nu^2
+    are linear in h = 1/nu^2 up to O(h^2), so their values at the two frequencies give c1 and c2 with an
+    error O(nu^-4)."""
+    nu = np.pi * (2 * np.array([iwmax - 1, iwmax - 2]) + 1) / beta
+    g = giw[[2 * iwmax - 1, 2 * iwmax - 2]]                   # (2, n)
+    h = 1 / nu ** 2
+    u = nu[:, None] ** 2 * g.real
+    v = nu[:, None] ** 3 * (g.imag + 1 / nu[:, None])
+    c1 = -(u[0] * h[1] - u[1] * h[0]) / (h[1] - h[0])
+    c2 = (v[0] * h[1] - v[1] * h[0]) / (h[1] - h[0])
+    return c1, c2
+
+
+def bubble_loc_tail(giw, beta, iwmax, K, iwb):
+    """The local bubble's frequency sum beyond the large range, nu outside [-K, K-1]:
+    -1/beta sum g_i(nu) g_j(nu - omega) on the diagonal i1 = i*n + j, with g expanded to third order in
+    1/(i nu) (the dropped terms are O(nu^-5), whose symmetric sum is O(K^-4) in units of beta/2pi)."""
+    S = tail_sums(beta, K, iwb)
+    c1, c2 = giw_moments(giw, beta, iwmax)
+    n = len(c1)
+    t = np.zeros(n * n, dtype=np.complex128)
+    for i in range(n):
+        for j in range(n):
+            t[i * n + j] = -(S[1, 1] + c1[i] * S[2, 1] + c1[j] * S[1, 2] + c2[i] * S[3, 1]
+                             + c1[i] * c1[j] * S[2, 2] + c2[j] * S[1, 3]) / beta
+    return np.diag(t)
+
+
 class HotPath:
     """The per-(q,omega) loop body of main.F90:486-799 over given inputs."""
 
@@ -182,9 +249,16 @@ class HotPath:
         self.beta = inp.beta
         self.off = inp.iwmax
         self.kq = kmq_table(wl.nk, inp.q_data)
+        # bubble sums over nu in [lo, hi]: the BSE box, or with the debug option "Bubble" every frequency
+        # for which G(nu - omega) exists (iwstart/iwstop, config_module.f90:467-477)
+        self.bubble_range = getattr(wl, "bubble_range", 0)
+        if self.bubble_range:
+            self.lo, self.hi = -inp.iwmax + self.iwb, inp.iwmax - self.iwb - 1
+        else:
+            self.lo, self.hi = -self.iwf, self.iwf - 1
         # frequencies used by G(k, nu) and G(k-q, nu-omega)
-        self.glo = -self.iwf - self.iwb
-        self.ghi = self.iwf + self.iwb - 1
+        self.glo = self.lo - self.iwb
+        self.ghi = self.hi + self.iwb
         self._gtab = None
         self.bse_inverse = bse_inverse or np.linalg.inv
 
@@ -243,6 +317,18 @@ class HotPath:
             out[:, :, d] = c.reshape(self.n2, self.n2) / self.nkp
         return out
 
+    def chi0q_outer(self, iq, iwb):
+        """sum of chi0^q(nu) over the large range outside the box (main.F90:644-655 with the debug option
+        "Bubble": those frequencies enter bubble_nl only)."""
+        kq = self.kq[:, iq]
+        acc = np.zeros((self.n2, self.n2), dtype=np.complex128)
+        for iwf in range(self.lo, self.hi + 1):
+            if -self.iwf <= iwf < self.iwf:
+                continue
+            c = -self.beta * np.einsum("Kil,Kkj->ijlk", self.G(iwf), self.G(iwf - iwb)[kq], optimize=True)
+            acc += c.reshape(self.n2, self.n2) / self.nkp
+        return acc
+
     # ---------------------------------------------------------------- A9
     def calc_chi_qw(self, x, y):
         """beta^-2 sum_{i3,nu} X(:,{i3,nu}) Y(i3,:,nu) (susc_module.f90:34-56)."""
@@ -258,7 +344,8 @@ class HotPath:
         chi0w = np.zeros((n2, n2, nf), dtype=np.complex128)
         for d in range(nf):
             chi0w[np.arange(n2), np.arange(n2), d] = w[:, d]
-        bubble_loc_tmp = chi0w.sum(axis=2) / self.beta ** 2
+        big = sum(self.chi0_loc(iwf, iwb) for iwf in range(self.lo, self.hi + 1))   # main.F90:513-520
+        bubble_loc_tmp = np.diag(big) / self.beta ** 2
         out = {"winv": winv, "chi0w": chi0w, "bubble_loc_tmp": bubble_loc_tmp}
         for r, cl in (("d", chi_loc_d), ("m", chi_loc_m)):
             a = 0.5 * (cl + cl.T)                        # main.F90:529-537
@@ -280,7 +367,10 @@ class HotPath:
         if chi0q is None:
             chi0q = self.chi0q(iq, iwb)
         res = {"chi0q": chi0q}
-        bubble_nl = chi0q.sum(axis=2) / self.beta ** 2 - loc["bubble_loc_tmp"]   # main.F90:663
+        bsum = chi0q.sum(axis=2)
+        if self.bubble_range:
+            bsum = bsum + self.chi0q_outer(iq, iwb)
+        bubble_nl = bsum / self.beta ** 2 - loc["bubble_loc_tmp"]   # main.F90:663
         chi0nl = chi0q - loc["chi0w"]
         res["bubble_nl"] = bubble_nl
         chi = {r: self.calc_chi_qw(loc["gammaw_" + r], chi0nl) for r in ("d", "m")}  # main.F90:666-667
@@ -413,6 +503,9 @@ class HotPath:
                 loc = self.local_omega(iwb, chi_loc_fn(iwb, "dens"), chi_loc_fn(iwb, "magn"))
                 if iq == 0:
                     out["bubble_loc"][iwb] = loc["bubble_loc_tmp"]
+                    if self.bubble_range == 2:
+                        out["bubble_loc"][iwb] = out["bubble_loc"][iwb] + bubble_loc_tail(
+                            self.inp.giw, self.beta, self.off, self.hi + 1, iwb)
                     out["chi_loc_d"][iwb] = loc["chi_loc_d"]
                     out["chi_loc_m"][iwb] = loc["chi_loc_m"]
             if do_eom and iwb == 0 and iq == 0:                    # main.F90:635-637

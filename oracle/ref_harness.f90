@@ -14,7 +14,10 @@
 ! arrives per q point in the compound index read_vq (interaction_module.f90:
 ! 39-123) produces.  Header flags: bit 0 do_vq, bit 1 k_path_eom (the EOM k list
 ! is not the whole grid: kq_ind_eom from index_kq_eom, main.F90:366-369), bit 2
-! do_eom off (a q path, which the reference runs without the EOM, main.F90:341-347).
+! do_eom off (a q path, which the reference runs without the EOM, main.F90:341-347),
+! bit 3 the debug option "Bubble" (iwstart/iwstop over the large range as
+! config_module.f90:467-477 sets them; chi0w and chi0q kept on the box only, as
+! main.F90:518, 656 do).
 !
 ! Usage: adga_ref_harness <input.bin> <output.bin>
 program adga_ref_harness
@@ -52,8 +55,6 @@ program adga_ref_harness
   nkp = nkpx*nkpy*nkpz
   ndim2 = ndim*ndim
   maxdim = ndim2*2*iwfmax_small
-  iwstart = -iwfmax_small
-  iwstop = iwfmax_small-1
   nw = 2*iwbmax_small+1
   verbose = .false.
   debug = .false.
@@ -64,6 +65,13 @@ program adga_ref_harness
   kpath = iand(flags, 2) /= 0
   do_chi = .true.
   do_eom = iand(flags, 4) == 0
+  if (iand(flags, 8) /= 0) then                        ! config_module.f90:467-477
+     iwstart = -iwmax+iwbmax_small
+     iwstop = iwmax-iwbmax_small-1
+  else
+     iwstart = -iwfmax_small
+     iwstop = iwfmax_small-1
+  endif
   nonlocal = .true.
   text_output = .false.
   allocate(hk(ndim,ndim,nkp), siw(-iwmax:iwmax-1,ndim), giw(-iwmax:iwmax-1,ndim), dc(2,ndim))
@@ -141,7 +149,7 @@ program adga_ref_harness
         do iwf = iwstart, iwstop
            call get_chi0_loc(iwf, iwb, kk)
            loc_tmp(1:ndim2,1:ndim2) = loc_tmp(1:ndim2,1:ndim2) + kk
-           chi0w(:,:,iwf) = kk
+           if (-iwfmax_small .le. iwf .and. iwf .le. iwfmax_small-1) chi0w(:,:,iwf) = kk
         enddo
         loc_tmp = loc_tmp/(beta**2)
         if (iq .eq. 1) bub_loc(:,:,iwb) = bub_loc(:,:,iwb) + loc_tmp(1:ndim2,1:ndim2)
@@ -191,7 +199,7 @@ program adga_ref_harness
         enddo
         kk = kk/dble(nkp)
         bub_nl(:,:,p) = bub_nl(:,:,p) + kk
-        chi0q(:,:,iwf) = kk
+        if (-iwfmax_small .le. iwf .and. iwf .le. iwfmax_small-1) chi0q(:,:,iwf) = kk
      enddo
      bub_nl(:,:,p) = bub_nl(:,:,p)/(beta**2) - loc_tmp(1:ndim2,1:ndim2)
      chi0nl = chi0q - chi0w

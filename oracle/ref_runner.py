@@ -36,9 +36,13 @@ def run(inp, points, chi_loc_fn, threads=1, timeout=3600, exe=None, args=()):
     npts = len(points)
     nkp_eom = len(inp.k_data_eom)
     wused = sorted({p[0] for p in points})
-    # header flags: bit 0 V(q), bit 1 EOM k list other than the grid in order, bit 2 no EOM (q path)
+    # header flags: bit 0 V(q), bit 1 EOM k list other than the grid in order, bit 2 no EOM (q path),
+    # bit 3 the bubble over the large range (debug option "Bubble"); the tail has no reference counterpart
     kpath = nkp_eom != int(np.prod(wl.nk)) or not np.array_equal(inp.k_data_eom, np.arange(1, nkp_eom + 1))
-    flags = (1 if inp.v_q is not None else 0) | (2 if kpath else 0) | (0 if wl.do_eom else 4)
+    br = getattr(wl, "bubble_range", 0)
+    if br == 2:
+        raise ValueError("the reference has no Matsubara tail (bubble_range 2)")
+    flags = (1 if inp.v_q is not None else 0) | (2 if kpath else 0) | (0 if wl.do_eom else 4) | (8 if br else 0)
     with tempfile.TemporaryDirectory() as td:
         fi = os.path.join(td, "in.bin")
         fo = os.path.join(td, "out.bin")

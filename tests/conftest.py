@@ -12,6 +12,13 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path through the C ABI)")
+    # the oracle's linear algebra is many small matrices: one BLAS thread per process is the fastest, and
+    # several test processes with a full BLAS thread pool each oversubscribe the cores by far
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1, user_api="blas")
+    except ImportError:
+        pass
 
 
 def rel_err(a, ref):

@@ -36,6 +36,8 @@ FIXTURES = {
     "t2g_k6_qpath": ("t2g_k6_qpath", None),  # chi along a q path (no EOM)
     "two_k4_vq": ("two_k4_vq", None),        # V(q): density BSE term, Hartree/Fock, V terms of the EOM
     "t2g_k4_vq": ("t2g_k4_vq", None),        # V(q) with the EOM on a reordered k list
+    "t2g_k4_bub": ("t2g_k4_bub", None),      # bubble over the large frequency range (debug option Bubble)
+    "two_k4_bub": ("two_k4_bub", None),      # the same, with the EOM on a reordered k list
 }
 
 
@@ -45,6 +47,8 @@ def input_digest(inp, points):
         h.update(np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).tobytes())
     # the q and EOM k lists and V(q) enter only when they differ from the defaults, so the digests of the
     # fixtures written before these inputs existed stay valid
+    if inp.wl.bubble_range:
+        h.update(np.int32(inp.wl.bubble_range).tobytes())
     if inp.wl.q_list != "vol" or inp.wl.k_eom != "all" or inp.v_q is not None:
         h.update(np.ascontiguousarray(inp.q_data, dtype=np.int32).tobytes())
         h.update(np.ascontiguousarray(inp.k_data_eom, dtype=np.int32).tobytes())

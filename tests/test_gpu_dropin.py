@@ -16,7 +16,8 @@ from adga_amd import synth
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "oracle", "_ref", "adga_dropin_harness")
 
-CASES = ["sc1_k8", "two_k4", "t2g_k4", "d5_k4", "t2g_k4_keom", "t2g_k6_qpath", "two_k4_vq", "t2g_k4_vq"]
+CASES = ["sc1_k8", "two_k4", "t2g_k4", "d5_k4", "t2g_k4_keom", "t2g_k6_qpath", "two_k4_vq", "t2g_k4_vq", "t2g_k4_bub",
+         "two_k4_bub"]
 
 
 @pytest.mark.parametrize("name", CASES)

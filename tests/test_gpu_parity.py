@@ -47,7 +47,7 @@ MODES = [(1, 1), (0, 0), (1, 0), (0, 1)]  # (bubble, eom): 1 = k-grid DFT, 0 = d
 @pytest.mark.parametrize("modes", MODES, ids=lambda m: f"bubble{m[0]}-eom{m[1]}")
 @pytest.mark.parametrize("precision", ["fp64", "refined"])
 @pytest.mark.parametrize("name", ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8", "t2g_k4_keom", "t2g_k6_qpath",
-                                  "two_k4_vq", "t2g_k4_vq"])
+                                  "two_k4_vq", "t2g_k4_vq", "t2g_k4_bub", "two_k4_bub"])
 def test_gpu_matches_reference_golden(name, precision, modes):
     from golden.make_golden import FIXTURES
     api = _api()

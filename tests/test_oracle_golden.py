@@ -9,7 +9,8 @@ from adga_amd import synth
 import adga_oracle
 from golden.make_golden import FIXTURES, input_digest
 
-CASES = ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8", "t2g_k4_keom", "t2g_k6_qpath", "two_k4_vq", "t2g_k4_vq"]
+CASES = ["sc1_k4", "two_k4", "t2g_k4", "d5_k4", "sc1_k8", "t2g_k4_keom", "t2g_k6_qpath", "two_k4_vq", "t2g_k4_vq",
+         "t2g_k4_bub", "two_k4_bub"]
 
 
 @pytest.mark.parametrize("name", CASES)
