@@ -21,7 +21,9 @@ EXPORTS = ["adga_ctx_create", "adga_set_option", "adga_set_local_omega", "adga_r
            "adga_set_tangents", "adga_set_local_omega_tangent", "adga_run_points_tangent", "adga_get_sigma_tangent",
            "adga_bubble_q", "adga_chi0_loc", "adga_chi_qw", "adga_eom_dynamic", "adga_eom_dmft", "adga_eom_static",
            "adga_get_gkiw", "adga_get_ndmft", "adga_add_siw_dmft",
-           "adga_inverse_matrix_z_noalloc"]
+           "adga_inverse_matrix_z_noalloc",
+           "adga_lambda_create", "adga_lambda_sums", "adga_lambda_target", "adga_lambda_solve", "adga_lambda_chi",
+           "adga_lambda_last_error", "adga_lambda_destroy"]
 
 
 class adga_dims(C.Structure):
@@ -70,6 +72,13 @@ def _load():
         "adga_get_nfock": ([_vp, _dp], C.c_int),
         "adga_get_sigma_g_loc": ([_vp, _dp, _dp, _dp], C.c_int),
         "adga_get_ndga": ([_vp, _dp, _dp, _dp], C.c_int),
+        "adga_lambda_create": ([C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, C.POINTER(_vp)], C.c_int),
+        "adga_lambda_sums": ([_vp, C.c_int, _dp, _dp, _dp], C.c_int),
+        "adga_lambda_target": ([_vp, _dp], C.c_int),
+        "adga_lambda_solve": ([_vp, C.c_double, C.c_double, C.c_int, _dp, _ip], C.c_int),
+        "adga_lambda_chi": ([_vp, C.c_double, _dp], C.c_int),
+        "adga_lambda_last_error": ([_vp, C.c_char_p, C.c_int], C.c_int),
+        "adga_lambda_destroy": ([_vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

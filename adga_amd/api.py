@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import ADGA_OK, ADGA_PREC_FP64, ADGA_PREC_REFINED, lib
 
-__all__ = ["AdgaError", "Context", "run_loop", "run_loop_tangent", "inverse_matrix", "PRECISION"]
+__all__ = ["AdgaError", "Context", "Lambda", "run_loop", "run_loop_tangent", "inverse_matrix", "PRECISION"]
 
 PRECISION = {"fp64": ADGA_PREC_FP64, "refined": ADGA_PREC_REFINED}
 
@@ -334,3 +334,74 @@ def inverse_matrix(m):
     if rc != ADGA_OK:
         raise AdgaError(rc, f"ERROR in ZGETRF (info={ierr.value})" if rc == _lib.ADGA_E_SINGULAR else "inverse failed")
     return a, ierr.value
+
+
+class Lambda:
+    """lambda correction of the final susceptibility on the device (adga_lambda_*; no reference counterpart).
+    chi_qw (n2, n2, nqp*(2*iwbmax+1)) over the omega-major (omega, q) list and chi_loc (n2, n2, 2*iwbmax+1),
+    both the final ones (with the bubbles); the physical susceptibility is sum_{i,j} chi[(i,i), (j,j)]."""
+
+    def __init__(self, chi_qw, chi_loc, beta, nqp):
+        n2 = chi_qw.shape[0]
+        n = int(round(n2 ** 0.5))
+        nw = chi_loc.shape[2]
+        if chi_qw.shape != (n2, n2, nqp * nw) or n * n != n2:
+            raise ValueError("chi_qw must be (n^2, n^2, nqp * n_omega) with chi_loc (n^2, n^2, n_omega)")
+        self.nqp, self.nw, self.beta = nqp, nw, float(beta)
+        self._cq = _zf(chi_qw)
+        self._cl = _zf(chi_loc)
+        h = C.c_void_p()
+        rc = lib.adga_lambda_create(n, nqp, (nw - 1) // 2, self.beta, _dptr(self._cq), _dptr(self._cl), C.byref(h))
+        if rc != ADGA_OK:
+            raise AdgaError(rc, "adga_lambda_create failed")
+        self.h = h
+        self._cq = self._cl = None
+
+    def _check(self, rc):
+        if rc != ADGA_OK:
+            buf = C.create_string_buffer(512)
+            lib.adga_lambda_last_error(self.h, buf, 512)
+            raise AdgaError(rc, buf.value.decode(errors="replace"))
+
+    def sums(self, lambdas):
+        """(S(lambda), dS/dlambda) for each lambda, complex arrays."""
+        lam = np.ascontiguousarray(np.atleast_1d(lambdas), dtype=np.float64)
+        s = np.zeros(lam.size, dtype=np.complex128)
+        ds = np.zeros_like(s)
+        self._check(lib.adga_lambda_sums(self.h, lam.size, lam.ctypes.data_as(_lib._dp), _dptr(s), _dptr(ds)))
+        return s, ds
+
+    def target(self):
+        t = C.c_double()
+        self._check(lib.adga_lambda_target(self.h, C.byref(t)))
+        return t.value
+
+    def solve(self, target=None, tol=1e-14, maxit=100):
+        """(lambda, iterations) with Re S(lambda) = target (the local sum rule by default)."""
+        t = self.target() if target is None else float(target)
+        lam = C.c_double()
+        it = C.c_int()
+        self._check(lib.adga_lambda_solve(self.h, t, tol, maxit, C.byref(lam), C.byref(it)))
+        return lam.value, it.value
+
+    def chi(self, lam):
+        out = np.zeros(self.nqp * self.nw, dtype=np.complex128)
+        self._check(lib.adga_lambda_chi(self.h, float(lam), _dptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.adga_lambda_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libadga_amd.so")
 SOURCES = ["local.hip", "bse.hip", "lu.hip", "eom.hip", "kconv.hip", "onep.hip", "vq.hip", "modules.hip", "tangent.hip",
-           "tail.hip", "inverse_noalloc.hip", "api.cpp", "api_modules.cpp", "api_tangent.cpp"]
+           "tail.hip", "lambda.hip", "inverse_noalloc.hip", "api.cpp", "api_modules.cpp", "api_tangent.cpp",
+           "api_lambda.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -137,5 +137,10 @@ void launch_tail(const Geom& g, int iwb, int K, const z64* giw, const z64* bublo
 void launch_bubble_outer(const Geom& g, int iwb, int lo, int hi, const z64* tw, const z64* ghat, z64* P, z64* out,
                          hipStream_t s);
 void launch_bubble_outer_add(const Geom& g, int npts, const int* qidx, const z64* all, z64* bub_out, hipStream_t s);
+// lambda correction sums (lambda.hip)
+void launch_chi_phys(int n, long npts, const z64* chi, z64* out, hipStream_t s);
+void launch_lambda_sums(long npts, int nl, int nblk, double scale, const double* lams, const z64* chi, z64* part,
+                        z64* out, hipStream_t s);
+void launch_lambda_apply(long npts, double lam, const z64* chi, z64* out, hipStream_t s);
 
 }  // namespace adga

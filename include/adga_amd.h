@@ -346,6 +346,40 @@ int adga_inverse_matrix_z(int n, double *m, int *ierr);
  */
 int adga_inverse_matrix_z_noalloc(int n, double *m, int *ierr);
 
+/*
+ * lambda correction of the final susceptibility (no counterpart in the reference, which has no
+ * lambda correction).  The physical susceptibility of a point is the sum over i, j of the reduced
+ * susceptibility's entries chi[(i,i), (j,j)] (those src/hdf5_module.f90:1479-1495 writes); the corrected
+ * one is chi^lambda = 1/(1/chi + lambda); lambda is fixed by the sum rule
+ *   Re S(lambda) = target,  S(lambda) = 1/(beta nqp) sum over every (q, omega) of chi^lambda,
+ * with the local sum 1/beta sum_omega chi_loc as the usual target.  The sums run in FP64 on the device.
+ */
+typedef struct adga_lambda adga_lambda;
+
+/*
+ * chi_qw(ndim^2, ndim^2, nqp*(2*iwbmax+1)): the final susceptibility of every point of the omega-major
+ * (omega, q) list (chi + bubble_nl + chi_loc + bubble_loc, the array main.F90 gathers as chi_qw_full,
+ * src/main.F90:1047-1063); chi_loc(ndim^2, ndim^2, 2*iwbmax+1): the final local susceptibility
+ * (chi_loc + bubble_loc, src/main.F90:935-936).  ADGA_E_ARG if no omega = 0 point has a positive
+ * physical susceptibility (lambda then has no lower bound).
+ */
+int adga_lambda_create(int ndim, int nqp, int iwbmax, double beta, const double *chi_qw, const double *chi_loc,
+                       adga_lambda **out);
+/* S(lambda) and dS/dlambda = -1/(beta nqp) sum (chi^lambda)^2 (complex, interleaved) for nl lambdas */
+int adga_lambda_sums(adga_lambda *h, int nl, const double *lambdas, double *s, double *ds);
+/* target = Re 1/beta sum_omega chi_loc_phys(omega), the local sum rule */
+int adga_lambda_target(adga_lambda *h, double *target);
+/*
+ * lambda in (-1/max Re chi_phys(q, omega = 0), inf) with |Re S(lambda) - target| <= tol max(1, |target|)
+ * (Newton's method on the device sums, bisection where a step leaves the bracket); ADGA_E_NOCONV after
+ * maxit steps.  iters (may be NULL): the steps taken.
+ */
+int adga_lambda_solve(adga_lambda *h, double target, double tol, int maxit, double *lambda, int *iters);
+/* chi_phys^lambda of every point, (nqp*(2*iwbmax+1)) complex */
+int adga_lambda_chi(adga_lambda *h, double lambda, double *chi_phys);
+int adga_lambda_last_error(adga_lambda *h, char *buf, int len);
+int adga_lambda_destroy(adga_lambda *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,10 @@ shapes and index meaning (first index fastest, 0-based here):
 * Matsubara tail            no reference counterpart: the local bubble summed beyond the large
                             range with the asymptotic expansion of giw (`tail_sums`,
                             `bubble_loc_tail`)
+* lambda correction         no reference counterpart: the physical susceptibility of the
+                            reduced output (hdf5_module.f90:1479-1495, chi_(ii),(jj) summed over
+                            i, j) corrected as 1/(1/chi + lambda), lambda from the sum rule
+                            (`chi_phys`, `lambda_sums`, `lambda_solve`)
 
 Parity pins (tests/test_oracle_*.py): golden vectors written by the reference's
 own Fortran modules (oracle/_ref, built from the reference sources with
@@ -47,7 +51,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["HotPath", "inverse_small", "kmq_table", "get_giw", "get_nfock", "get_sigma_g_loc", "get_ndga",
-           "tail_sums", "giw_moments", "bubble_loc_tail"]
+           "tail_sums", "giw_moments", "bubble_loc_tail", "chi_phys", "lambda_sums", "lambda_solve"]
 
 
 def inverse_small(a):
@@ -229,6 +233,36 @@ def bubble_loc_tail(giw, beta, iwmax, K, iwb):
             t[i * n + j] = -(S[1, 1] + c1[i] * S[2, 1] + c1[j] * S[1, 2] + c2[i] * S[3, 1]
                              + c1[i] * c1[j] * S[2, 2] + c2[j] * S[1, 3]) / beta
     return np.diag(t)
+
+
+def chi_phys(chi):
+    """sum_{i,j} chi[(i,i), (j,j)] of chi (n2, n2, ...): the reduced susceptibility's entries
+    (hdf5_module.f90:1479-1495: i2 = i1, i4 = i3) summed over both orbitals."""
+    n = int(round(np.sqrt(chi.shape[0])))
+    d = np.arange(n) * n + np.arange(n)
+    return chi[np.ix_(d, d)].sum(axis=(0, 1))
+
+
+def lambda_sums(chi, beta, nqp, lam):
+    """S(lambda) = 1/(beta nqp) sum over every point of chi^lambda, chi^lambda = 1/(1/chi + lambda), and
+    dS/dlambda = -1/(beta nqp) sum (chi^lambda)^2, for the physical susceptibilities chi of the points."""
+    cl = chi / (1 + lam * chi)
+    return cl.sum() / (beta * nqp), -(cl * cl).sum() / (beta * nqp)
+
+
+def lambda_solve(chi, beta, nqp, target, chi0=None):
+    """lambda with Re S(lambda) = target by Brent's method (scipy.optimize.brentq) on (lambda_min, lambda_hi],
+    lambda_min = -1/max Re chi(omega = 0) the pole of the largest omega = 0 susceptibility (`chi0`; all
+    points if not given): Re S decreases monotonically from +inf there (for positive Re chi at omega = 0)."""
+    from scipy.optimize import brentq
+    c0 = chi if chi0 is None else chi0
+    lmin = -1.0 / np.max(c0.real)
+    f = lambda lam: lambda_sums(chi, beta, nqp, lam)[0].real - target  # noqa: E731
+    hi = 1.0
+    while f(hi) > 0:
+        hi *= 2.0
+    lo = lmin + 1e-12 * max(1.0, abs(lmin))
+    return brentq(f, lo, hi, xtol=1e-15, rtol=1e-15, maxiter=500)
 
 
 class HotPath:
