@@ -566,8 +566,11 @@ __global__ void __launch_bounds__(NT) k_lu_perm(int N, int* __restrict__ pivall)
 }
 
 // ---------------------------------------------------------------- the solve kernel
-// Y (+)= (P^T L U)^-1 R for the ns right-hand-side columns [blockIdx.y*ns, ...) of matrix
-// blockIdx.x, R and Y in FP64 with layout [matrix][row][nrhs].  The columns are gathered through
+// Y (+)= (P^T L U)^-1 R for the ns right-hand-side columns [g*ns, ...) of matrix m, R and Y in FP64 with
+// layout [matrix][row][nrhs].  When the columns do not fit the LDS at once they are split into ng groups,
+// each a workgroup that streams all of the matrix's factors; the groups of one matrix are therefore
+// dispatched on one XCD in consecutive slots (workgroup b runs on XCD b % 8), so that they read the same
+// factor rows at about the same time through that XCD's L2 instead of each from HBM.  The columns are gathered through
 // the permutation into LDS (in the factor precision) and stay there; the factors stream from HBM
 // once: block row by block row (P rows), the off-diagonal part as dot products against the
 // resident solution (16 lanes per row, shuffle reduction), the diagonal block through its stored
@@ -577,16 +580,21 @@ constexpr int SOLVE_LDS_BYTES = 80896;        // two workgroups per CU
 constexpr int SOLVE_LDS_BYTES_WIDE = 163840;  // one workgroup per CU (the whole LDS)
 
 template <typename T, int CG, int LDSB>
-__global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ldm, const cplx<T>* __restrict__ Mall,
-                                                const int* __restrict__ permall, const cplx<T>* __restrict__ invall,
-                                                const z64* __restrict__ R, z64* __restrict__ Y, int accumulate,
-                                                const int* __restrict__ active) {
+__global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng, int nmat, int xcd, int ldm,
+                                                const cplx<T>* __restrict__ Mall, const int* __restrict__ permall,
+                                                const cplx<T>* __restrict__ invall, const z64* __restrict__ R,
+                                                z64* __restrict__ Y, int accumulate, const int* __restrict__ active) {
   using C = cplx<T>;
   // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
   // columns without predicates
   __shared__ C x[LDSB / sizeof(C)];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
-  const int m = blockIdx.x, s0 = blockIdx.y * ns, nsg = min(ns, nrhs - s0);
+  // xcd: the XCD-grouped mapping; else workgroup b is group b % ng of matrix b / ng (few matrices: their
+  // groups spread over every XCD)
+  const int slot = xcd ? blockIdx.x >> 3 : blockIdx.x;
+  const int m = xcd ? (slot / ng) * 8 + (blockIdx.x & 7) : slot / ng, grp = slot % ng;
+  if (m >= nmat) return;  // the grid is rounded up to whole XCD rounds: the whole workgroup leaves
+  const int s0 = grp * ns, nsg = min(ns, nrhs - s0);
   if (active && !active[m]) return;  // the whole workgroup leaves: no barrier is reached
   const C* M = Mall + (long)m * N * ldm;
   const int* perm = permall + (long)m * N;
@@ -1052,9 +1060,10 @@ LuTimer::~LuTimer() {
 }
 
 // columns per group: one group when a solution of nrhs columns (padded to the kernel's column count)
-// fits the LDS at two workgroups per CU; else one group at one workgroup per CU when it fits the
-// whole LDS (the factors are then read once instead of once per group); else groups as wide as fit
-// at two workgroups per CU.  False when not even one column fits.
+// fits the LDS at two workgroups per CU; else one group on the whole LDS (one workgroup per CU) when it
+// fits there; else, with enough matrices to fill the GPU (>= 64), the whole LDS per workgroup when that
+// needs fewer groups (each group streams all the factors); else groups as wide as fit at two workgroups
+// per CU (few matrices: more groups, more workgroups in flight).  False when not even one column fits.
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active) {
@@ -1072,20 +1081,26 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
   int cg = pick(SOLVE_LDS_BYTES, true);
   bool wide = false;
   if (!cg) {
-    cg = pick(SOLVE_LDS_BYTES_WIDE, true);
-    wide = cg == 4 || cg == 9 || cg == 16;
-    if (!wide) cg = pick(SOLVE_LDS_BYTES, false);
+    const int narrow = pick(SOLVE_LDS_BYTES, false), wcg = pick(SOLVE_LDS_BYTES_WIDE, false);
+    const int gn = narrow ? (nrhs + narrow - 1) / narrow : 1 << 30, gw = wcg ? (nrhs + wcg - 1) / wcg : 1 << 30;
+    wide = gw < gn && (wcg == 2 || wcg == 4 || wcg == 9 || wcg == 16) && (gw == 1 || nmat >= 64 || !narrow);
+    cg = wide ? wcg : narrow;
   }
   if (!cg) return false;
   const int ns = std::min(nrhs, cg), ng = (nrhs + ns - 1) / ns;
-  dim3 grid(nmat, ng);
+  // one workgroup per (matrix, group); with several groups and enough matrices to fill the XCDs, the
+  // matrices are dealt round-robin over the 8 XCDs and each matrix's groups take consecutive slots of its
+  // XCD (k_lu_solve's mapping; the grid rounded up to whole rounds of 8)
+  const int xcd = (ng > 1 && nmat >= 64) ? 1 : 0;
+  dim3 grid((unsigned)(xcd ? 8 * ((nmat + 7) / 8) * ng : nmat * ng));
 #define ADGA_SOLVE_CASE(CGV, LDSB)                                                                              \
   if (cg == CGV) {                                                                                              \
-    hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ldm, M, perm, inv, R, Y, \
-                       accumulate ? 1 : 0, active);                                                             \
+    hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ng, nmat, xcd, ldm, M,   \
+                       perm, inv, R, Y, accumulate ? 1 : 0, active);                                            \
     return true;                                                                                                \
   }
   if (wide) {
+    ADGA_SOLVE_CASE(2, SOLVE_LDS_BYTES_WIDE)
     ADGA_SOLVE_CASE(4, SOLVE_LDS_BYTES_WIDE)
     ADGA_SOLVE_CASE(9, SOLVE_LDS_BYTES_WIDE)
     ADGA_SOLVE_CASE(16, SOLVE_LDS_BYTES_WIDE)

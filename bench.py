@@ -204,6 +204,71 @@ def level1_inverse(api, N, reps=3):
             "note": "one matrix per call (one workgroup per panel), hipMalloc/hipFree and a host transpose per call"}
 
 
+def shrinking_box(api, synth, batch=128, omegas=(0, 5, 20), warmup=1, steps=1):
+    """The 5-orbital d-shell workload (16^3 k, 41 bosonic frequencies) with the fermionic box shrinking
+    with |omega|, 2 x (80 - 2|omega|): one context with the largest box, each omega at its own BSE size
+    (adga_set_local_omega_box).  `steps` batches of `batch` q points per omega after `warmup` untimed ones,
+    refined and FP64; the full run (4096 q at each of the 41 omegas) is estimated from the measured time per
+    point, fitted as a + b N^3 over the three sizes."""
+    wl = synth.workload("d5_k16")
+    inp = synth.build_inputs(wl)
+    box = lambda w: 80 - 2 * abs(w)  # noqa: E731
+    res = {"workload": "d5_k16: 5-orbital d-shell, 16^3 k, box 2x(80-2|omega|), 41 bosonic frequencies",
+           "batch_points": batch}
+    for precision in ("refined", "fp64"):
+        per = {}
+        with api.Context(inp, batch=batch) as ctx:
+            for w in omegas:
+                # chi_loc of the omega's box: the central frequency block of the largest box's
+                o, nb = (wl.iwfmax_small - box(w)) * wl.n ** 2, 2 * box(w) * wl.n ** 2
+                xd, xm = (synth.chi_loc(inp, w, ch)[o:o + nb, o:o + nb] for ch in ("dens", "magn"))
+                ctx.set_local_omega(w, xd, xm, box=box(w))
+                q = 0
+                for s in range(warmup + steps):
+                    if s == warmup:
+                        t0 = time.perf_counter()
+                    iq = np.arange(q, q + batch, dtype=np.int32) + 1
+                    ctx.run_points(iq, precision)
+                    q += batch
+                dt = time.perf_counter() - t0
+                N = wl.n ** 2 * 2 * box(w)
+                per[w] = {"N": N, "points_per_s": round(steps * batch / dt, 3)}
+            ctx.get_sigma()
+        Ns = np.array([per[w]["N"] for w in omegas], dtype=float)
+        sec = np.array([1.0 / per[w]["points_per_s"] for w in omegas])
+        a, b = np.linalg.lstsq(np.stack([np.ones_like(Ns), Ns ** 3], axis=1), sec, rcond=None)[0]
+        allN = np.array([wl.n ** 2 * 2 * box(w) for w in range(-wl.iwbmax_small, wl.iwbmax_small + 1)], dtype=float)
+        full = float(np.sum(wl.nqp * (a + b * allN ** 3)))
+        res[precision] = {"per_omega": {str(w): per[w] for w in omegas},
+                          "full_run_estimate_s": round(full, 1),
+                          "full_run_points_per_s": round(wl.nqp * len(allN) / full, 3)}
+    res["refined_over_fp64"] = round(res["fp64"]["full_run_estimate_s"] / res["refined"]["full_run_estimate_s"], 3)
+    return res
+
+
+def gradient(synth):
+    """The differentiation workload (3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all 36288
+    points): the forward pass (complex128) and the backward pass (forward + tangent-linear along t, t', U)
+    of adga_amd.grad.ChiSigma, wall time each, after one untimed forward pass."""
+    import torch
+    from adga_amd import grad
+    wl = synth.workload("t2g_k12")
+    npts = (2 * wl.iwbmax_small + 1) * wl.nqp
+    grad.forward_arrays(wl)  # warm-up
+    theta = torch.tensor([wl.t, wl.tp, wl.U], dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    chi, sig = grad.ChiSigma.apply(theta, wl)
+    t1 = time.perf_counter()
+    loss = chi.real.sum() + sig.real.sum()
+    loss.backward()
+    t2 = time.perf_counter()
+    return {"workload": "t2g_k12: 3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all points",
+            "points": npts, "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "forward_points_per_s": round(npts / (t1 - t0), 3),
+            "backward_points_per_s": round(npts / (t2 - t1), 3), "directions": 3,
+            "grad": [float(x) for x in theta.grad]}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes` (reads and writes `nbytes` each), outside any timed region."""
     import torch
@@ -234,6 +299,8 @@ def main():
     ap.add_argument("--only-one-particle", action="store_true",
                     help="time only the one-particle routines (get_giw, get_nfock, get_sigma_g_loc, get_ndga) "
                          "and print their JSON object")
+    ap.add_argument("--no-shrinking-box", action="store_true", help="skip the shrinking-box (5-orbital) timing")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the gradient (forward + tangent) timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device and exit: a kernel of known HBM traffic whose "
                          "hardware byte counters scale those of the hot-path kernels")
@@ -331,6 +398,10 @@ def main():
         line["cpu_baseline"] = cpu_baseline(synth, inp, chi_cache, args.cpu_points, args.cpu_procs)
     if not args.no_one_particle:
         line["one_particle"] = one_particle(api, synth, cpu=not args.no_cpu_baseline)
+    if not args.no_shrinking_box:
+        line["shrinking_box"] = shrinking_box(api, synth)
+    if not args.no_gradient:
+        line["gradient"] = gradient(synth)
     print(json.dumps(line), flush=True)
     if line.get("parity_ok") is False:
         print("refined results differ from the plain FP64 path by more than 1e-10 (or points failed)",

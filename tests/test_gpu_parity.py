@@ -260,6 +260,35 @@ def test_gpu_five_orbital_large_box_against_oracle():
             assert e < 1e-10, (precision, k, e)
 
 
+def test_gpu_solve_column_groups_on_a_full_batch():
+    """64 points at N = 2000 (5 orbitals, 16^3 k, box 2x40) in one batch: 128 matrices, so the solves'
+    n^2 = 25 right-hand-side columns run in column groups on the whole LDS (complex64: 3 groups of 9,
+    complex128: 7 of 4), each matrix's groups on one XCD.  Two of the points against the oracle, every
+    point of the refined mode against the FP64 mode, and the FP64 mode's own residual."""
+    import adga_oracle
+    api = _api()
+    wl = synth.workload("d5_k16", iwfmax_small=40)
+    inp = synth.build_inputs(wl)
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    pts = [(0, q) for q in range(64)]
+    check = [0, 37]
+    print("oracle d5_k16 box 40 (N = 2000), 2 points", flush=True)
+    ref = adga_oracle.HotPath(inp).run([pts[i] for i in check], cl)
+    out = {}
+    for precision in ("fp64", "refined"):
+        print(f"GPU {precision}, 64 points in one batch", flush=True)
+        with api.Context(inp, batch=64) as ctx:
+            out[precision] = api.run_loop(ctx, pts, cl, precision)
+            st = ctx.stats()
+        assert (out[precision]["status"] == 0).all()
+        assert st["refine_resid"] <= 1e-12, st["refine_resid"]
+        for k in ("chi_d", "chi_m", "bubble_nl"):
+            e = rel_err(out[precision][k][:, :, check], ref[k])
+            assert e < 1e-10, (precision, k, e)
+    for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
+        assert rel_err(out["refined"][k], out["fp64"][k]) < 1e-10, k
+
+
 def test_gpu_refinement_hands_unconverged_matrices_to_fp64():
     """refine_maxit = 1 with a tolerance one complex64 correction cannot reach: every matrix is solved
     again in complex128, so the refined mode returns the plain FP64 path's results (the same kernels on
