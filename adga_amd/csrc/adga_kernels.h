@@ -48,15 +48,22 @@ struct LuTimer {
   double collect();
   ~LuTimer();
 };
+// streams an LU batch may be split over: lane 0 is the launch stream itself, lanes 1 .. n-1 the streams
+// s[1..], with the events that order them (all streams at the default priority)
+constexpr int LU_MAX_LANES = 4;
+struct LuLanes {
+  int n = 1;
+  hipStream_t s[LU_MAX_LANES] = {};
+  hipEvent_t lag[LU_MAX_LANES] = {}, join[LU_MAX_LANES] = {};
+};
 template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, size_t nops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
-               hipEvent_t lag = nullptr, hipEvent_t join = nullptr);
+               int* info, hipStream_t s, LuTimer* timer = nullptr, const LuLanes* lanes = nullptr);
 template <typename T>
 inline bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& ops, cplx<T>* M, int* piv,
-                      cplx<T>* inv, int* info, hipStream_t s, LuTimer* timer = nullptr, hipStream_t s2 = nullptr,
-                      hipEvent_t lag = nullptr, hipEvent_t join = nullptr) {
-  return launch_lu<T>(nmat, N, W, ldm, ops.data(), ops.size(), M, piv, inv, info, s, timer, s2, lag, join);
+                      cplx<T>* inv, int* info, hipStream_t s, LuTimer* timer = nullptr,
+                      const LuLanes* lanes = nullptr) {
+  return launch_lu<T>(nmat, N, W, ldm, ops.data(), ops.size(), M, piv, inv, info, s, timer, lanes);
 }
 // `active` (may be null): per-matrix flags; matrices with a zero flag are skipped (Y left as is)
 template <typename T>

@@ -293,9 +293,12 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     for (int i = 0; i < g.nkp_eom; ++i)
       if (k_data_eom[i] < 1 || k_data_eom[i] > g.nkp) throw HipError{hipErrorInvalidValue, "k_data_eom out of range"};
     CK(hipStreamCreate(&c->stream));
-    CK(hipStreamCreate(&c->stream2));
-    CK(hipEventCreateWithFlags(&c->ev_lag, hipEventDisableTiming));
-    CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    c->lanes.s[0] = c->stream;
+    for (int j = 1; j < adga::LU_MAX_LANES; ++j) CK(hipStreamCreate(&c->lanes.s[j]));
+    for (int j = 0; j < adga::LU_MAX_LANES; ++j) {
+      CK(hipEventCreateWithFlags(&c->lanes.lag[j], hipEventDisableTiming));
+      CK(hipEventCreateWithFlags(&c->lanes.join[j], hipEventDisableTiming));
+    }
     for (auto& e : c->ev) CK(hipEventCreate(&e));
     for (auto& e : c->ev_lu) CK(hipEventCreate(&e));
     hipStream_t s = c->stream;
@@ -355,7 +358,7 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
     return ADGA_OK;
   }
   if (!strcmp(key, "lu_streams")) {
-    if (!(v == 1 || v == 2)) return fail(c, ADGA_E_ARG, "lu_streams must be 1 or 2");
+    if (!(v >= 1 && v <= adga::LU_MAX_LANES && v == (int)v)) return fail(c, ADGA_E_ARG, "lu_streams must be 1 .. 4");
     c->lu_split = (int)v;
     return ADGA_OK;
   }
@@ -545,6 +548,8 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       double maxrel = 0.0;
       bool lu_ok;
       const std::vector<int>& ops = ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
+      adga::LuLanes lanes = c->lanes;
+      lanes.n = c->lu_split;
       relh.assign(nmat, 0.0);
       infoh.assign(nmat, 0);
       resolved.assign(nmat, 0);
@@ -554,8 +559,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, ops, (z64*)c->M, c->piv,
-                                        (z64*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
-                                        c->ev_lag, c->ev_join);
+                                        (z64*)c->inv, c->info, s, &c->lut, &lanes);
         CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->piv, (const z64*)c->inv,
@@ -573,8 +577,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
         lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, ops, (c32*)c->M, c->piv,
-                                       (c32*)c->inv, c->info, s, &c->lut, c->lu_split == 2 ? c->stream2 : nullptr,
-                                       c->ev_lag, c->ev_join);
+                                       (c32*)c->inv, c->info, s, &c->lut, &lanes);
         CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv,
@@ -844,9 +847,11 @@ int adga_ctx_destroy(adga_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (auto e : c->ev_lu)
     if (e) (void)hipEventDestroy(e);
-  if (c->ev_lag) (void)hipEventDestroy(c->ev_lag);
-  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  for (int j = 0; j < adga::LU_MAX_LANES; ++j) {
+    if (c->lanes.lag[j]) (void)hipEventDestroy(c->lanes.lag[j]);
+    if (c->lanes.join[j]) (void)hipEventDestroy(c->lanes.join[j]);
+    if (j > 0 && c->lanes.s[j]) (void)hipStreamDestroy(c->lanes.s[j]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ADGA_OK;

@@ -35,9 +35,8 @@ struct adga_ctx {
   Geom gw{};  // the current omega's geometry: g with the BSE box of that omega
   Geom eom_g{};  // geometry of the omega whose EOM is pending
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;          // second half of each LU batch (default priority)
-  hipEvent_t ev_lag = nullptr, ev_join = nullptr;
-  int lu_split = 2;                        // option "lu_streams": 2 splits each LU batch over two streams
+  adga::LuLanes lanes;                     // streams 1.. of the LU lanes (default priority) and their events
+  int lu_split = 2;                        // option "lu_streams": parts each LU batch is split into
   hipEvent_t ev[7] = {}, ev_lu[2] = {};    // stage and LU timing of adga_run_points
   std::string err;
   int refine_maxit = 12;

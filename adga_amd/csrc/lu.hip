@@ -982,58 +982,67 @@ bool lu_op(int nmat, int N, int W, int ldm, const int* op, cplx<T>* M, int* piv,
 // `lag` and `join` are events the caller owns.
 template <typename T>
 bool launch_lu(int nmat, int N, int W, int ldm, const int* ops, size_t nops, cplx<T>* M, int* piv, cplx<T>* inv,
-               int* info, hipStream_t s, LuTimer* timer, hipStream_t s2, hipEvent_t lag, hipEvent_t join) {
+               int* info, hipStream_t s, LuTimer* timer, const LuLanes* lanes) {
   if (panel_cfg(N) < 0 || N > PERM_MAX_N || nmat < 1) return false;
   const long mstride = (long)N * ldm, istride = (long)lu_inv_elems(N);
-  const bool split = s2 && lag && join && nmat >= 16;
-  const int na = split ? nmat / 2 : nmat, nb = nmat - na;
-  cplx<T>* Mb = M + (long)na * mstride;
-  cplx<T>* ib = inv + (long)na * istride;
-  int* pb = piv + (long)na * N;
-  int* fb = info + na;
+  // the batch in nl parts of about equal size (at least 8 matrices each), part j on lane j; after each join
+  // lane j starts its next operation when lane j-1 has finished it, so the parts run staggered by one
+  // operation and one part's latency-bound panel or thin update runs beside another's; the large Schur
+  // updates run on the whole batch on s alone, between a join of every lane and the next staggered start
+  const int nl = (lanes && lanes->n > 1) ? std::min(std::min(lanes->n, LU_MAX_LANES), std::max(1, nmat / 8)) : 1;
+  hipStream_t st[LU_MAX_LANES];
+  int m0[LU_MAX_LANES + 1];
+  for (int j = 0; j < nl; ++j) {
+    st[j] = j == 0 ? s : lanes->s[j];
+    m0[j] = (int)((long)nmat * j / nl);
+  }
+  m0[nl] = nmat;
+  auto part_op = [&](int j, const int* op, LuTimer* t) {
+    const int a = m0[j], cnt = m0[j + 1] - a;
+    return lu_op<T>(cnt, N, W, ldm, op, M + (long)a * mstride, piv + (long)a * N, inv + (long)a * istride,
+                    info + a, st[j], t);
+  };
   auto sync = [](hipEvent_t e, hipStream_t from, hipStream_t to) {
     return hipEventRecord(e, from) == hipSuccess && hipStreamWaitEvent(to, e, 0) == hipSuccess;
   };
   hipLaunchKernelGGL(k_lu_init, dim3((nmat + 255) / 256), dim3(256), 0, s, nmat, info);
-  if (split && !sync(lag, s, s2)) return false;
   bool uinv = false;
   auto launch_uinv = [&]() {
     const int np = (N + P - 1) / P;
-    hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, na), dim3(64), 0, s, N, ldm, mstride, istride, M, inv);
-    if (split) hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, nb), dim3(64), 0, s2, N, ldm, mstride, istride, Mb, ib);
+    for (int j = 0; j < nl; ++j)
+      hipLaunchKernelGGL(k_lu_uinv<T>, dim3(np, m0[j + 1] - m0[j]), dim3(64), 0, st[j], N, ldm, mstride, istride,
+                         M + (long)m0[j] * mstride, inv + (long)m0[j] * istride);
     uinv = true;
   };
-  bool relag = true;  // the second stream waits for the first one's next operation
+  bool relag = true;  // the next operation starts staggered (lane j after lane j-1)
   for (size_t o = 0; o + 8 <= nops; o += 8) {
     const int* op = &ops[o];
     if (op[0] == OP_TRSM_U && !uinv) launch_uinv();  // the upper triangular products need U11^-1
     const bool big = op[0] == OP_GEMM && op[2] - op[1] > 64 && op[4] - op[3] > 64;
-    if (split && big) {
-      if (!sync(join, s2, s)) return false;
+    if (nl > 1 && big) {
+      for (int j = 1; j < nl; ++j)
+        if (!sync(lanes->join[j], st[j], s)) return false;
       if (!lu_op<T>(nmat, N, W, ldm, op, M, piv, inv, info, s, timer)) return false;
-      if (!sync(lag, s, s2)) return false;
       relag = true;
       continue;
     }
-    if (!lu_op<T>(na, N, W, ldm, op, M, piv, inv, info, s, timer)) return false;
-    if (split) {
-      if (relag && !sync(lag, s, s2)) return false;
-      relag = false;
-      if (!lu_op<T>(nb, N, W, ldm, op, Mb, pb, ib, fb, s2, nullptr)) return false;
+    for (int j = 0; j < nl; ++j) {
+      if (j > 0 && relag && !sync(lanes->lag[j - 1], st[j - 1], st[j])) return false;
+      if (!part_op(j, op, j == 0 ? timer : nullptr)) return false;
     }
+    relag = false;
   }
   if (!uinv) launch_uinv();
-  hipLaunchKernelGGL(k_lu_perm, dim3(na), dim3(NT), 0, s, N, piv);
-  if (split) {
-    hipLaunchKernelGGL(k_lu_perm, dim3(nb), dim3(NT), 0, s2, N, pb);
-    if (!sync(join, s2, s)) return false;
-  }
+  for (int j = 0; j < nl; ++j)
+    hipLaunchKernelGGL(k_lu_perm, dim3(m0[j + 1] - m0[j]), dim3(NT), 0, st[j], N, piv + (long)m0[j] * N);
+  for (int j = 1; j < nl; ++j)
+    if (!sync(lanes->join[j], st[j], s)) return false;
   return true;
 }
 template bool launch_lu<float>(int, int, int, int, const int*, size_t, c32*, int*, c32*, int*, hipStream_t,
-                               LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
+                               LuTimer*, const LuLanes*);
 template bool launch_lu<double>(int, int, int, int, const int*, size_t, z64*, int*, z64*, int*, hipStream_t,
-                                LuTimer*, hipStream_t, hipEvent_t, hipEvent_t);
+                                LuTimer*, const LuLanes*);
 
 void LuTimer::mark(hipStream_t s) {
   if (used == ev.size()) {

@@ -79,8 +79,9 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
  * k-grid DFT, cached per omega; 2: DFT when a batch has >= 64 points or the cache is valid, the
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
  * G by k-grid DFT when omega changes or the self-energy is read, the default), "lu_streams" (1: each
- * batch's LU on one stream; 2: the batch split in two halves on two streams, the second one
- * operation behind, the large Schur updates on the whole batch; the default; results are identical),
+ * batch's LU on one stream; 2 .. 4: the batch split into that many parts on as many streams, each one
+ * operation behind the previous, the large Schur updates on the whole batch; 2 is the default; results
+ * are identical),
  * "bubble_range" (0: bubble_loc and bubble_nl sum chi0 over the vertex box, the default; 1: over every
  * nu for which G(nu - omega) exists, [-iwmax + iwbmax_small, iwmax - iwbmax_small), the reference's
  * debug option "Bubble" (src/config_module.f90:467-477); 2: as 1, and the bubble_loc returned by

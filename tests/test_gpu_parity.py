@@ -221,21 +221,23 @@ def test_gpu_strong_vertex_pivoting_against_oracle(name, noise, over):
             assert e < 1e-10, (precision, k, e)
 
 
-def test_gpu_two_stream_lu_is_identical(t2g_full):
-    """lu_streams = 2 (the batch's LU split over two streams, the large Schur updates on the whole
-    batch) runs the same kernels on the same data: results equal the one-stream run exactly, in
-    both modes, on a 96-point batch (above the split threshold of 16 matrices)."""
+def test_gpu_multi_stream_lu_is_identical(t2g_full):
+    """lu_streams = 2, 3, 4 (the batch's LU split into that many parts on as many streams, staggered by
+    one operation, the large Schur updates on the whole batch) run the same kernels on the same data:
+    results equal the one-stream run exactly, in both modes, on a 98-point batch (196 matrices: parts
+    of 98, 65/65/66 and 49 matrices)."""
     api = _api()
     inp = t2g_full
-    pts = [(1, q) for q in range(5, 8000, 83)]
+    pts = [(1, q) for q in range(5, 8000, 82)]
     cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
     for precision in ("fp64", "refined"):
         res = {}
-        for ls in (1, 2):
+        for ls in (1, 2, 3, 4):
             with api.Context(inp, lu_streams=ls) as ctx:
                 res[ls] = api.run_loop(ctx, pts, cl, precision)
-        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
-            assert np.array_equal(res[1][k], res[2][k]), (precision, k)
+        for ls in (2, 3, 4):
+            for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
+                assert np.array_equal(res[1][k], res[ls][k]), (precision, ls, k)
 
 
 def test_gpu_five_orbital_large_box_against_oracle():
