@@ -34,7 +34,9 @@ void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z
 void launch_gamq_z(const Geom& g, int npts, const z64* S, z64* Z, hipStream_t s);
 size_t lu_inv_elems(int N);
 std::vector<int> lu_ops_factor(int N, int W);
-// the same list written into buf (at most cap ints); returns the number of ints the list needs
+// factor + forward elimination of the right-hand-side columns [N, W) only (no backward substitution)
+std::vector<int> lu_ops_factor_fwd(int N, int W);
+// the same list as lu_ops_factor written into buf (at most cap ints); returns the number of ints the list needs
 size_t lu_ops_factor_into(int N, int W, int* buf, size_t cap);
 // Events around the launches of the largest GEMM tile shape (the factorisation's dominant kernel):
 // mark() records one event; collect() (after the stream has completed) returns the summed
@@ -65,10 +67,19 @@ inline bool launch_lu(int nmat, int N, int W, int ldm, const std::vector<int>& o
                       const LuLanes* lanes = nullptr) {
   return launch_lu<T>(nmat, N, W, ldm, ops.data(), ops.size(), M, piv, inv, info, s, timer, lanes);
 }
-// `active` (may be null): per-matrix flags; matrices with a zero flag are skipped (Y left as is)
+// `active` (may be null): per-matrix flags; matrices with a zero flag are skipped (Y left as is).
+// from_m: only the backward substitution, of the right-hand sides the factorisation carried along in the
+// columns [N, N + nrhs) of M (lu_ops_factor_fwd); R is then not read
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
-                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active = nullptr);
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active = nullptr,
+                     bool from_m = false);
+// whether the nrhs columns of a solution fit one workgroup's LDS at once (elem bytes per complex element):
+// else every solve streams the factors once per column group
+bool lu_solve_one_group(int N, int nrhs, size_t elem);
+// the right-hand sides B ([matrix][row][nrhs], FP64) into the columns [N, N + nrhs) of each matrix of M
+template <typename T>
+void launch_rhs_into_m(int nmat, int N, int nrhs, int ldm, const z64* B, cplx<T>* M, hipStream_t s);
 // The non-local interaction's rank-n2 term of the density BSE matrix (main.F90:710-719): W holds, per
 // point, the N x n2 rows (2/beta^2) chi0^q(nu) V(q); opgd = 1 + gamma^w_d; Q is n2 x n2 per point of scratch.
 struct VqTerm {

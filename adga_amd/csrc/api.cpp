@@ -124,6 +124,24 @@ const std::vector<int>& ensure_ops(adga_ctx* c, int prec) {
   return ops;
 }
 
+// whether the current omega's factorisation carries the n^2 right-hand sides as columns [N, N + n^2) of each
+// matrix: their forward elimination then rides along in the factorisation's updates, and the first solve is
+// only the backward substitution.  Chosen where the solution's columns do not fit the LDS at once, so that
+// every solve pass streams the factors once per column group (at N = 4000: +7% points/s refined); where
+// they fit, the extra columns cost the factorisation about what the forward pass saves (measured at
+// N = 1080: 143 -> 148 ms per refined factorisation of 4096 matrices), and they are left out.
+bool carry_rhs(const adga_ctx* c, int prec) {
+  const Geom& g = c->gw;
+  return !adga::lu_solve_one_group(g.N, g.n2, prec == 0 ? sizeof(z64) : sizeof(c32)) && c->ldm >= g.N + g.n2;
+}
+
+const std::vector<int>& ensure_ops_rhs(adga_ctx* c, int prec) {
+  const int N = c->gw.N;
+  std::vector<int>& ops = c->ops_r[prec][N];
+  if (ops.empty()) ops = adga::lu_ops_factor_fwd(N, N + c->gw.n2);
+  return ops;
+}
+
 // FP64 re-solve of the points `pts` (batch positions) whose complex64 refinement did not converge:
 // their S and right-hand sides are gathered, the density and magnetic matrices rebuilt in complex128,
 // factored, solved and checked, and the solutions scattered back into the batch's Y.  Updates the
@@ -282,7 +300,8 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     g.nkp_eom = d->nkp_eom;
     g.beta = d->beta;
     g.mu = d->mu;
-    c->ldm = (g.N + 15) / 16 * 16;
+    // N columns, and the n^2 right-hand sides when the factorisation carries them (carry_rhs)
+    c->ldm = (g.N + (adga::lu_solve_one_group(g.N, g.n2, sizeof(z64)) ? 0 : g.n2) + 15) / 16 * 16;
     c->gw = g;
     c->eom_g = g;
     c->q0.resize(g.nqp);
@@ -547,7 +566,10 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       int iters = 0;
       double maxrel = 0.0;
       bool lu_ok;
-      const std::vector<int>& ops = ensure_ops(c, precision == ADGA_PREC_FP64 ? 0 : 1);
+      const int prec = precision == ADGA_PREC_FP64 ? 0 : 1;
+      const bool rhs = carry_rhs(c, prec);
+      const std::vector<int>& ops = rhs ? ensure_ops_rhs(c, prec) : ensure_ops(c, prec);
+      const int W = rhs ? g.N + n2 : g.N;
       adga::LuLanes lanes = c->lanes;
       lanes.n = c->lu_split;
       relh.assign(nmat, 0.0);
@@ -556,14 +578,15 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       if (precision == ADGA_PREC_FP64) {
         adga::launch_build<double>(g, nb, c->ldm, c->S, c->cd, c->cm, (z64*)c->M, s);
         if (vtp) adga::launch_vq_build<double>(g, nb, c->ldm, vt, (z64*)c->M, s);
+        if (rhs) adga::launch_rhs_into_m<double>(nmat, g.N, n2, c->ldm, c->Bp, (z64*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
-        lu_ok = adga::launch_lu<double>(nmat, g.N, g.N, c->ldm, ops, (z64*)c->M, c->piv,
-                                        (z64*)c->inv, c->info, s, &c->lut, &lanes);
+        lu_ok = adga::launch_lu<double>(nmat, g.N, W, c->ldm, ops, (z64*)c->M, c->piv, (z64*)c->inv, c->info, s,
+                                        &c->lut, &lanes);
         CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<double>(nmat, g.N, n2, c->ldm, (const z64*)c->M, c->piv, (const z64*)c->inv,
-                                                c->Bp, c->Y, false, s);
+                                                rhs ? nullptr : c->Bp, c->Y, false, s, nullptr, rhs);
         if (lu_ok) {
           // one FP64 residual of the solution: reports the achieved accuracy and non-finite solves
           adga::launch_residual(g, nb, c->S, c->cd, c->cm, c->Bp, c->Y, c->Z, c->R, c->rel, s, vtp);
@@ -574,14 +597,15 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       } else {
         adga::launch_build<float>(g, nb, c->ldm, c->S, c->cd, c->cm, (c32*)c->M, s);
         if (vtp) adga::launch_vq_build<float>(g, nb, c->ldm, vt, (c32*)c->M, s);
+        if (rhs) adga::launch_rhs_into_m<float>(nmat, g.N, n2, c->ldm, c->Bp, (c32*)c->M, s);
         CK(hipEventRecord(ev[3], s));
         CK(hipEventRecord(c->ev_lu[0], s));
-        lu_ok = adga::launch_lu<float>(nmat, g.N, g.N, c->ldm, ops, (c32*)c->M, c->piv,
-                                       (c32*)c->inv, c->info, s, &c->lut, &lanes);
+        lu_ok = adga::launch_lu<float>(nmat, g.N, W, c->ldm, ops, (c32*)c->M, c->piv, (c32*)c->inv, c->info, s,
+                                       &c->lut, &lanes);
         CK(hipEventRecord(c->ev_lu[1], s));
         if (lu_ok)
           lu_ok = adga::launch_lu_solve<float>(nmat, g.N, n2, c->ldm, (const c32*)c->M, c->piv, (const c32*)c->inv,
-                                               c->Bp, c->Y, false, s);
+                                               rhs ? nullptr : c->Bp, c->Y, false, s, nullptr, rhs);
         if (lu_ok) {
           CK(hipMemcpyAsync(infoh.data(), c->info, sizeof(int) * nmat, hipMemcpyDeviceToHost, s));
           iters = refine(c, nb, relh, infoh, resolved);

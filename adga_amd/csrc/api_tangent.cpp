@@ -121,11 +121,11 @@ int adga_set_tangents(adga_ctx* c, int ndir, const double* dhk, const double* ds
   if (!c || ndir < 1 || ndir > 16 || !dhk || !dsiw || !dgiw || !dmu || !du || !dut) return ADGA_E_ARG;
   if (c->eom_pending) return fail(c, ADGA_E_STATE, "tangents cannot change while an omega is pending (adga_get_sigma)");
   if (c->bubble_range != 0) return fail(c, ADGA_E_STATE, "the derivatives are formed with the bubble on the box");
-  return guarded(c, [&]() -> int {
+  if (c->g.n > 5) return fail(c, ADGA_E_ARG, "derivatives: orbital count above 5");
+  free_tangent(c);  // ndir = 0 until every buffer below is in place
+  const int rc = guarded(c, [&]() -> int {
     const Geom& g = c->g;
     hipStream_t s = c->stream;
-    free_tangent(c);
-    c->ndir = ndir;
     const size_t nh = (size_t)g.n2 * g.nkp * ndir, ns = (size_t)2 * g.iwmax * g.n * ndir;
     DevBuf<> dh(nh), dsw(ns);
     upload(dh.p, dhk, nh, s);
@@ -141,6 +141,7 @@ int adga_set_tangents(adga_ctx* c, int ndir, const double* dhk, const double* ds
     dalloc(&c->dgt, (size_t)g.nw * g.nkp * g.n2 * ndir);
     if (!adga::launch_dgtable(g, ndir, c->gt, dh.p, dsw.p, c->dmu, c->dgt, s))
       return fail(c, ADGA_E_ARG, "derivatives: orbital count above 5");
+    c->ndir = ndir;  // the per-direction buffers below are sized by it
     const size_t loc = (size_t)g.n2 * g.nf * ndir, cc = (size_t)g.N * g.N * ndir, pm = (size_t)g.n2 * g.N * ndir;
     dalloc(&c->t_dw, loc);
     dalloc(&c->t_dwinv, loc);
@@ -155,6 +156,8 @@ int adga_set_tangents(adga_ctx* c, int ndir, const double* dhk, const double* ds
     CK(hipStreamSynchronize(s));
     return ADGA_OK;
   });
+  if (rc != ADGA_OK) free_tangent(c);  // no half-set directions
+  return rc;
 }
 
 int adga_set_local_omega_tangent(adga_ctx* c, const double* dxd, const double* dxm, double* dchid_out,

@@ -72,6 +72,7 @@ struct adga_ctx {
   FallbackBufs fb;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision
   std::map<int, std::vector<int>> ops_f[2];  // LU operation lists (host) per precision and BSE size
+  std::map<int, std::vector<int>> ops_r[2];  // the same with the n^2 right-hand sides carried along
   // accumulated self-energies
   z64 *sig_nl = nullptr, *sig_dmft = nullptr, *sig_hf = nullptr;
   // k-grid convolutions: twiddles, Ghat table, per-omega bubble cache, pending EOM of one omega

@@ -583,7 +583,8 @@ template <typename T, int CG, int LDSB>
 __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng, int nmat, int xcd, int ldm,
                                                 const cplx<T>* __restrict__ Mall, const int* __restrict__ permall,
                                                 const cplx<T>* __restrict__ invall, const z64* __restrict__ R,
-                                                z64* __restrict__ Y, int accumulate, const int* __restrict__ active) {
+                                                z64* __restrict__ Y, int accumulate, const int* __restrict__ active,
+                                                int from_m) {
   using C = cplx<T>;
   // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
   // columns without predicates
@@ -600,15 +601,23 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
   const int* perm = permall + (long)m * N;
   const int nb = (N + P - 1) / P;
   const C* inv = invall + (long)m * nb * 2 * P * P;
-  const z64* Rm = R + (long)m * N * nrhs + s0;
+  const z64* Rm = R ? R + (long)m * N * nrhs + s0 : nullptr;  // not read with from_m
   z64* Ym = Y + (long)m * N * nrhs + s0;
-  for (int e = t; e < (N + P) * CG; e += NT) {
-    const int j = e / CG, s = e % CG;
-    x[e] = (j < N && s < nsg) ? ccast<T>(Rm[(long)perm[j] * nrhs + s]) : mk<T>(0, 0);
+  if (from_m) {
+    // the right-hand sides the factorisation carried along: L^-1 P B in the columns [N, N + nrhs) of M
+    for (int e = t; e < (N + P) * CG; e += NT) {
+      const int j = e / CG, s = e % CG;
+      x[e] = (j < N && s < nsg) ? M[(long)j * ldm + N + s0 + s] : mk<T>(0, 0);
+    }
+  } else {
+    for (int e = t; e < (N + P) * CG; e += NT) {
+      const int j = e / CG, s = e % CG;
+      x[e] = (j < N && s < nsg) ? ccast<T>(Rm[(long)perm[j] * nrhs + s]) : mk<T>(0, 0);
+    }
   }
   __syncthreads();
   constexpr int SU = 8, G = 16 * SU;  // loads per row per group; k covered by one group of 16 lanes
-  for (int pass = 0; pass < 2; ++pass) {  // 0: forward with unit-lower L, 1: backward with U
+  for (int pass = from_m ? 1 : 0; pass < 2; ++pass) {  // 0: forward with unit-lower L, 1: backward with U
     for (int q = 0; q < nb; ++q) {
       const int ib = pass == 0 ? q : nb - 1 - q;
       const int r0 = ib * P, r1 = min(N, r0 + P);
@@ -838,6 +847,14 @@ std::vector<int> lu_ops_factor(int N, int W) {
   Ops ops;
   rec_lu(ops, N, W, 0, N, true);
   if (W > N) rec_trsm_u(ops, 0, N, N, W);
+  return ops.v;
+}
+
+// factor + forward elimination of the appended right-hand sides (L^-1 P B in columns [N, W); the backward
+// substitution is left to the solve kernel, which then reads only U)
+std::vector<int> lu_ops_factor_fwd(int N, int W) {
+  Ops ops;
+  rec_lu(ops, N, W, 0, N, true);
   return ops.v;
 }
 
@@ -1075,7 +1092,7 @@ LuTimer::~LuTimer() {
 // per CU (few matrices: more groups, more workgroups in flight).  False when not even one column fits.
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
-                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active) {
+                     const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active, bool from_m) {
   static const int cgs[] = {1, 2, 4, 9, 16};
   auto pick = [&](int ldsb, bool whole) {
     const int cap = ldsb / (int)sizeof(cplx<T>) / (N + P);  // columns that fit, with P zero rows
@@ -1105,7 +1122,7 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
 #define ADGA_SOLVE_CASE(CGV, LDSB)                                                                              \
   if (cg == CGV) {                                                                                              \
     hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ng, nmat, xcd, ldm, M,   \
-                       perm, inv, R, Y, accumulate ? 1 : 0, active);                                            \
+                       perm, inv, R, Y, accumulate ? 1 : 0, active, from_m ? 1 : 0);                           \
     return true;                                                                                                \
   }
   if (wide) {
@@ -1123,9 +1140,38 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
 #undef ADGA_SOLVE_CASE
   return false;
 }
+// whether every nrhs columns of a solution fit one workgroup's LDS (the whole LDS) in complex elements of
+// `elem` bytes: then a solve streams the factors once; else once per column group
+bool lu_solve_one_group(int N, int nrhs, size_t elem) {
+  static const int cgs[] = {1, 2, 4, 9, 16};
+  for (int c : cgs)
+    if (c >= nrhs && (size_t)c * (N + P) * elem <= (size_t)SOLVE_LDS_BYTES_WIDE) return true;
+  return false;
+}
+
 template bool launch_lu_solve<float>(int, int, int, int, const c32*, const int*, const c32*, const z64*, z64*, bool,
-                                     hipStream_t, const int*);
+                                     hipStream_t, const int*, bool);
 template bool launch_lu_solve<double>(int, int, int, int, const z64*, const int*, const z64*, const z64*, z64*, bool,
-                                      hipStream_t, const int*);
+                                      hipStream_t, const int*, bool);
+
+// M[m][r][N + s] = B[m][r][s] (in the factor precision): the right-hand sides appended to each matrix, for
+// the factorisation to carry along
+template <typename T>
+__global__ void k_rhs_into_m(int nmat, int N, int nrhs, int ldm, const z64* __restrict__ B, cplx<T>* __restrict__ M) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)N * nrhs;
+  if (idx >= per * nmat) return;
+  const long m = idx / per;
+  const int r = (int)((idx % per) / nrhs), sidx = (int)(idx % nrhs);
+  M[m * N * ldm + (long)r * ldm + N + sidx] = ccast<T>(B[idx]);
+}
+
+template <typename T>
+void launch_rhs_into_m(int nmat, int N, int nrhs, int ldm, const z64* B, cplx<T>* M, hipStream_t s) {
+  const long tot = (long)nmat * N * nrhs;
+  hipLaunchKernelGGL(k_rhs_into_m<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nmat, N, nrhs, ldm, B, M);
+}
+template void launch_rhs_into_m<float>(int, int, int, int, const z64*, c32*, hipStream_t);
+template void launch_rhs_into_m<double>(int, int, int, int, const z64*, z64*, hipStream_t);
 
 }  // namespace adga
