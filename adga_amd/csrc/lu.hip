@@ -574,13 +574,13 @@ __global__ void __launch_bounds__(NT) k_lu_perm(int N, int* __restrict__ pivall)
 // the permutation into LDS (in the factor precision) and stay there; the factors stream from HBM
 // once: block row by block row (P rows), the off-diagonal part as dot products against the
 // resident solution (16 lanes per row, shuffle reduction), the diagonal block through its stored
-// inverse.  Wave w owns rows 8w .. 8w+7 of a block row, lane l rows 8w + l/16 and 8w + 4 + l/16 at
-// columns l%16 + 16j.
+// inverse.  With 512 threads wave w owns rows 4w .. 4w+3 of a block row, lane l row 4w + l/16 at columns
+// l%16 + 16j (with 256 threads: rows 8w .. 8w+7, lane l rows 8w + l/16 and 8w + 4 + l/16).
 constexpr int SOLVE_LDS_BYTES = 80896;        // two workgroups per CU
 constexpr int SOLVE_LDS_BYTES_WIDE = 163840;  // one workgroup per CU (the whole LDS)
 
-template <typename T, int CG, int LDSB>
-__global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng, int nmat, int xcd, int ldm,
+template <typename T, int CG, int LDSB, int NTS>
+__global__ void __launch_bounds__(NTS) k_lu_solve(int N, int nrhs, int ns, int ng, int nmat, int xcd, int ldm,
                                                 const cplx<T>* __restrict__ Mall, const int* __restrict__ permall,
                                                 const cplx<T>* __restrict__ invall, const z64* __restrict__ R,
                                                 z64* __restrict__ Y, int accumulate, const int* __restrict__ active,
@@ -589,6 +589,9 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
   // x[j * CG + s]: rows [N, N + P) and columns [nsg, CG) are zeros, so every loop below runs over CG
   // columns without predicates
   __shared__ C x[LDSB / sizeof(C)];
+  // ROWS rows per 16-lane group: 4 waves x 8 rows (256 threads) or 8 waves x 4 rows (512) cover a block row
+  constexpr int ROWS = NTS == 256 ? 2 : 1;
+  static_assert(NTS == 256 || NTS == 512, "256 or 512 threads");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, rs = lane >> 4, ks = lane & 15;
   // xcd: the XCD-grouped mapping; else workgroup b is group b % ng of matrix b / ng (few matrices: their
   // groups spread over every XCD)
@@ -605,12 +608,12 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
   z64* Ym = Y + (long)m * N * nrhs + s0;
   if (from_m) {
     // the right-hand sides the factorisation carried along: L^-1 P B in the columns [N, N + nrhs) of M
-    for (int e = t; e < (N + P) * CG; e += NT) {
+    for (int e = t; e < (N + P) * CG; e += NTS) {
       const int j = e / CG, s = e % CG;
       x[e] = (j < N && s < nsg) ? M[(long)j * ldm + N + s0 + s] : mk<T>(0, 0);
     }
   } else {
-    for (int e = t; e < (N + P) * CG; e += NT) {
+    for (int e = t; e < (N + P) * CG; e += NTS) {
       const int j = e / CG, s = e % CG;
       x[e] = (j < N && s < nsg) ? ccast<T>(Rm[(long)perm[j] * nrhs + s]) : mk<T>(0, 0);
     }
@@ -623,7 +626,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
       const int r0 = ib * P, r1 = min(N, r0 + P);
       const int k0 = pass == 0 ? 0 : r1, k1 = pass == 0 ? r0 : N;
       if (k1 > k0) {
-        const int ra = r0 + wave * 8 + rs, rb = ra + 4;
+        const int ra = r0 + wave * 4 * ROWS + rs, rb = ROWS == 2 ? ra + 4 : N;  // rb unused with one row
         const C* La = M + (long)min(ra, N - 1) * ldm;
         const C* Lb = M + (long)min(rb, N - 1) * ldm;
         C acc0[CG], acc1[CG];
@@ -634,7 +637,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
 #pragma unroll
           for (int u = 0; u < SU; ++u) {
             d0[u] = La[kb + 16 * u];
-            d1[u] = Lb[kb + 16 * u];
+            if constexpr (ROWS == 2) d1[u] = Lb[kb + 16 * u];
           }
         };
         auto use = [&](const C* d0, const C* d1, int kb) {
@@ -645,7 +648,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
             for (int s = 0; s < CG; ++s) {
               const C xv = xr[s];
               acc0[s] = cfma(d0[u], xv, acc0[s]);
-              acc1[s] = cfma(d1[u], xv, acc1[s]);
+              if constexpr (ROWS == 2) acc1[s] = cfma(d1[u], xv, acc1[s]);
             }
           }
         };
@@ -671,7 +674,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
           for (int u = 0; u < SU; ++u) {
             const bool in = k + 16 * u < k1;
             pa0[u] = in ? La[k + 16 * u] : mk<T>(0, 0);
-            pa1[u] = in ? Lb[k + 16 * u] : mk<T>(0, 0);
+            if constexpr (ROWS == 2) pa1[u] = in ? Lb[k + 16 * u] : mk<T>(0, 0);
           }
 #pragma unroll
           for (int u = 0; u < SU; ++u) {
@@ -681,7 +684,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
               for (int s = 0; s < CG; ++s) {
                 const C xv = xr[s];
                 acc0[s] = cfma(pa0[u], xv, acc0[s]);
-                acc1[s] = cfma(pa1[u], xv, acc1[s]);
+                if constexpr (ROWS == 2) acc1[s] = cfma(pa1[u], xv, acc1[s]);
               }
             }
           }
@@ -692,25 +695,28 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
           for (int off = 8; off > 0; off >>= 1) {
             acc0[s].re += __shfl_xor(acc0[s].re, off);
             acc0[s].im += __shfl_xor(acc0[s].im, off);
-            acc1[s].re += __shfl_xor(acc1[s].re, off);
-            acc1[s].im += __shfl_xor(acc1[s].im, off);
+            if constexpr (ROWS == 2) {
+              acc1[s].re += __shfl_xor(acc1[s].re, off);
+              acc1[s].im += __shfl_xor(acc1[s].im, off);
+            }
           }
         }
         if (ks == 0) {
 #pragma unroll
           for (int s = 0; s < CG; ++s) {
             if (ra < r1) x[ra * CG + s] = csub(x[ra * CG + s], acc0[s]);
-            if (rb < r1) x[rb * CG + s] = csub(x[rb * CG + s], acc1[s]);
+            if (ROWS == 2 && rb < r1) x[rb * CG + s] = csub(x[rb * CG + s], acc1[s]);
           }
         }
       }
       __syncthreads();
       // diagonal block: x_b <- D^-1 x_b with D^-1 the stored inverse (P x P, zero beyond the block)
       const C* Di = inv + (long)ib * 2 * P * P + (pass == 0 ? 0 : P * P);
-      C o[2];
+      constexpr int H = (P * CG + NTS - 1) / NTS;  // entries of the block per thread
+      C o[H];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = t + h * NT;
+      for (int h = 0; h < H; ++h) {
+        const int e = t + h * NTS;
         o[h] = mk<T>(0, 0);
         if (e < P * CG) {
           const int r = e / CG, s = e % CG;
@@ -733,8 +739,8 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
       }
       __syncthreads();
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = t + h * NT;
+      for (int h = 0; h < H; ++h) {
+        const int e = t + h * NTS;
         if (e < P * CG) {
           const int r = e / CG;
           if (r0 + r < r1) x[r0 * CG + e] = o[h];
@@ -743,7 +749,7 @@ __global__ void __launch_bounds__(NT) k_lu_solve(int N, int nrhs, int ns, int ng
       __syncthreads();
     }
   }
-  for (int e = t; e < N * nsg; e += NT) {
+  for (int e = t; e < N * nsg; e += NTS) {
     const int j = e / nsg, s = e % nsg;
     const C v = x[j * CG + s];
     z64 y = mk((double)v.re, (double)v.im);
@@ -1090,6 +1096,8 @@ LuTimer::~LuTimer() {
 // fits there; else, with enough matrices to fill the GPU (>= 64), the whole LDS per workgroup when that
 // needs fewer groups (each group streams all the factors); else groups as wide as fit at two workgroups
 // per CU (few matrices: more groups, more workgroups in flight).  False when not even one column fits.
+constexpr int SOLVE_NT = 512;
+
 template <typename T>
 bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const int* perm, const cplx<T>* inv,
                      const z64* R, z64* Y, bool accumulate, hipStream_t s, const int* active, bool from_m) {
@@ -1119,10 +1127,13 @@ bool launch_lu_solve(int nmat, int N, int nrhs, int ldm, const cplx<T>* M, const
   // XCD (k_lu_solve's mapping; the grid rounded up to whole rounds of 8)
   const int xcd = (ng > 1 && nmat >= 64) ? 1 : 0;
   dim3 grid((unsigned)(xcd ? 8 * ((nmat + 7) / 8) * ng : nmat * ng));
+  // 512 threads per workgroup (8 waves of 4 rows): twice the loads in flight per CU of 256 threads (4 waves
+  // of 8 rows) under the same LDS limit of workgroups per CU; measured +0.7% points/s refined and +1.5% FP64
+  // at N = 1080, +2% (N = 4000) to +6% (N = 2000) with column groups
 #define ADGA_SOLVE_CASE(CGV, LDSB)                                                                              \
   if (cg == CGV) {                                                                                              \
-    hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB>), grid, dim3(NT), 0, s, N, nrhs, ns, ng, nmat, xcd, ldm, M,   \
-                       perm, inv, R, Y, accumulate ? 1 : 0, active, from_m ? 1 : 0);                           \
+    hipLaunchKernelGGL((k_lu_solve<T, CGV, LDSB, SOLVE_NT>), grid, dim3(SOLVE_NT), 0, s, N, nrhs, ns, ng, nmat, \
+                       xcd, ldm, M, perm, inv, R, Y, accumulate ? 1 : 0, active, from_m ? 1 : 0);               \
     return true;                                                                                                \
   }
   if (wide) {
