@@ -301,6 +301,9 @@ def main():
                          "and print their JSON object")
     ap.add_argument("--no-shrinking-box", action="store_true", help="skip the shrinking-box (5-orbital) timing")
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient (forward + tangent) timing")
+    ap.add_argument("--only-shrinking-box", action="store_true",
+                    help="time only the shrinking-box workload and print its JSON object")
+    ap.add_argument("--only-gradient", action="store_true", help="time only the gradient pass and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device and exit: a kernel of known HBM traffic whose "
                          "hardware byte counters scale those of the hot-path kernels")
@@ -316,6 +319,12 @@ def main():
     from adga_amd import api, synth
     if args.only_one_particle:
         print(json.dumps({"one_particle": one_particle(api, synth, cpu=not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.only_shrinking_box:
+        print(json.dumps({"shrinking_box": shrinking_box(api, synth)}), flush=True)
+        return
+    if args.only_gradient:
+        print(json.dumps({"gradient": gradient(synth)}), flush=True)
         return
     wl = synth.workload(args.workload)
     inp = synth.build_inputs(wl)
