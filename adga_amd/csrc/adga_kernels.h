@@ -146,9 +146,22 @@ void dft3(const Geom& g, long nfield, int sign, const z64* tw, z64* a, z64* b, h
 void launch_ghat(const Geom& g, const z64* tw, const z64* gt, z64* tmp, z64* ghat, hipStream_t s);
 void launch_bubble_all(const Geom& g, int iwb, const z64* tw, const z64* ghat, z64* P, z64* out, hipStream_t s);
 void launch_bubble_gather(const Geom& g, int npts, const int* qidx, const z64* all, z64* chi0q, hipStream_t s);
+// the bubble's derivative for every q of an omega from the transforms of G and dG (product rule)
+void launch_bubble_all_tangent(const Geom& g, int iwb, const z64* tw, const z64* ghat, const z64* dghat, z64* P,
+                               z64* out, hipStream_t s);
 void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s);
+// derivatives carried by the EOM flush: per direction d, the accumulator of dm_d on the q grid (macc + d*nm),
+// the k-grid transform of dG_d (dghat + d*ng) and d Sigma_nl (sigma + d*nsig); work: nm scratch
+struct EomTangent {
+  int ndir = 0;
+  z64* macc = nullptr;
+  const z64* dghat = nullptr;
+  z64* work = nullptr;
+  z64* sigma = nullptr;
+  size_t nm = 0, ng = 0, nsig = 0;
+};
 void launch_eom_flush(const Geom& g, int iwb, const int* keom, const z64* tw, const z64* ghat, z64* macc, z64* work,
-                      z64* sigma, hipStream_t s);
+                      z64* sigma, hipStream_t s, const EomTangent* tan = nullptr);
 // bubble beyond the box: large range and Matsubara tail (tail.hip)
 void launch_bubloc_range(const Geom& g, int iwb, int lo, int hi, const z64* giw, z64* bubloc, hipStream_t s);
 void launch_tail(const Geom& g, int iwb, int K, const z64* giw, const z64* bubloc, z64* out, hipStream_t s);

@@ -110,9 +110,23 @@ void widen_gtable(adga_ctx* c) {
 
 void eom_flush(adga_ctx* c) {
   if (!c->eom_pending) return;
-  adga::launch_eom_flush(c->eom_g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream);
+  const size_t nm = (size_t)c->g.iwf * c->g.n4 * c->g.nkp;
+  adga::EomTangent tan;
+  if (c->ndir > 0 && c->t_macc) {
+    tan.ndir = c->ndir;
+    tan.macc = c->t_macc;
+    tan.dghat = c->t_dghat;
+    tan.work = c->t_mwork;
+    tan.sigma = c->t_sig;
+    tan.nm = nm;
+    tan.ng = (size_t)c->g.nw * c->g.n2 * c->g.nkp;
+    tan.nsig = (size_t)c->g.n2 * c->g.nf * c->g.nkp_eom;
+  }
+  adga::launch_eom_flush(c->eom_g, c->eom_iwb, c->keom, c->tw, c->ghat, c->macc, c->mwork, c->sig_nl, c->stream,
+                         tan.ndir ? &tan : nullptr);
   CK(hipGetLastError());
-  CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)c->g.iwf * c->g.n4 * c->g.nkp, c->stream));
+  CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, c->stream));
+  if (tan.ndir) CK(hipMemsetAsync(c->t_macc, 0, sizeof(z64) * nm * tan.ndir, c->stream));
   c->eom_pending = false;
 }
 
@@ -266,6 +280,11 @@ int refine(adga_ctx* c, int nb, std::vector<double>& relh, std::vector<int>& inf
 }
 
 }  // namespace
+
+namespace adga_host {
+void conv_tables(adga_ctx* c) { ensure_conv(c); }
+void flush_eom(adga_ctx* c) { eom_flush(c); }
+}  // namespace adga_host
 
 extern "C" {
 
@@ -834,8 +853,11 @@ int adga_reset_sigma(adga_ctx* c) {
     CK(hipMemsetAsync(c->sig_hf, 0, sizeof(z64) * g.n2 * g.nkp_eom, c->stream));
     if (c->t_sig) CK(hipMemsetAsync(c->t_sig, 0, sizeof(z64) * g.n2 * g.nf * g.nkp_eom * c->ndir, c->stream));
     if (c->macc) CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp, c->stream));
+    if (c->t_macc)
+      CK(hipMemsetAsync(c->t_macc, 0, sizeof(z64) * (size_t)g.iwf * g.n4 * g.nkp * c->ndir, c->stream));
     c->eom_pending = false;
     c->bub_valid = false;  // the next batch forms the bubble of its omega again
+    c->t_bub_valid = false;
     CK(hipStreamSynchronize(c->stream));
     return ADGA_OK;
   });

@@ -44,9 +44,11 @@ namespace adga_host {
 void free_tangent(adga_ctx* c) {
   for (z64** p : {&c->dgt, &c->dgiw, &c->du, &c->dut, &c->t_dw, &c->t_dwinv, &c->t_dcd, &c->t_dcm, &c->t_dgamd,
                   &c->t_dgamm, &c->t_dchi0q, &c->t_dS, &c->t_Z1, &c->t_Z2, &c->t_B, &c->t_dY, &c->t_deta, &c->t_dgamq,
-                  &c->t_dmt, &c->t_dchi, &c->t_dbub, &c->t_sig})
+                  &c->t_dmt, &c->t_dchi, &c->t_dbub, &c->t_sig, &c->t_macc, &c->t_dghat, &c->t_mwork,
+                  &c->t_dbub_all, &c->t_bub_p})
     dfree(*p);
   dfree(c->dmu);
+  c->t_bub_valid = false;
   c->ndir = 0;
   c->t_cap = 0;
   c->t_omega = false;
@@ -66,9 +68,31 @@ void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>
     const z64 *dw = c->t_dw + dir * loc, *dwinv = c->t_dwinv + dir * loc;
     const z64 *dcd = c->t_dcd + dir * cc, *dcm = c->t_dcm + dir * cc;
     const z64 *dgamd = c->t_dgamd + dir * pm, *dgamm = c->t_dgamm + dir * pm;
-    // dchi0q = bubble(dG(k, nu), G(k - q, nu - omega)) + bubble(G(k, nu), dG(k - q, nu - omega))
-    adga::launch_bubble(g, c->iwb, nb, c->qidx, dgt, c->t_dchi0q, s, c->gt, false);
-    adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->t_dchi0q, s, dgt, true);
+    // dchi0q = bubble(dG(k, nu), G(k - q, nu - omega)) + bubble(G(k, nu), dG(k - q, nu - omega)): for every q
+    // of the omega at once by k-grid convolution (formed again, not stored from the forward pass, and kept
+    // for the omega's next batches), or as direct sums per batch, as the forward bubble's mode chooses
+    const bool t_hit = c->t_bub_valid && c->t_bub_iwb == c->iwb && c->t_bub_iwf == g.iwf;
+    if (c->t_dghat && (c->bubble_mode == 1 || (c->bubble_mode == 2 && (t_hit || nb >= 64)))) {
+      const size_t nall = (size_t)G.nf * G.n4 * G.nkp;  // sized for the largest box
+      if (!t_hit) {
+        if (dir == 0) {
+          if (!c->t_dbub_all) {
+            dalloc(&c->t_dbub_all, nall * c->ndir);
+            dalloc(&c->t_bub_p, nall);
+          }
+          for (int d = 0; d < c->ndir; ++d)
+            adga::launch_bubble_all_tangent(g, c->iwb, c->tw, c->ghat, c->t_dghat + d * tab, c->t_bub_p,
+                                            c->t_dbub_all + d * nall, s);
+          c->t_bub_valid = true;
+          c->t_bub_iwb = c->iwb;
+          c->t_bub_iwf = g.iwf;
+        }
+      }
+      adga::launch_bubble_gather(g, nb, c->qidx, c->t_dbub_all + dir * nall, c->t_dchi0q, s);
+    } else {
+      adga::launch_bubble(g, c->iwb, nb, c->qidx, dgt, c->t_dchi0q, s, c->gt, false);
+      adga::launch_bubble(g, c->iwb, nb, c->qidx, c->gt, c->t_dchi0q, s, dgt, true);
+    }
     adga::launch_dsmat(g, nb, c->chi0q, c->t_dchi0q, c->winv, dwinv, c->t_dS, s);
     // d eta: M d eta^T = C^T Z1 + dC^T Z2, solved with the complex128 factors of the forward solve
     adga::launch_tangent_rhs(g, nb, c->S, c->t_dS, c->cd, c->cm, dcd, dcm, c->opgd, c->opgm, dgamd, dgamm, c->Y,
@@ -90,10 +114,16 @@ void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>
     adga::launch_mtot(g, nb, c->du + (size_t)dir * G.n4, c->dut + (size_t)dir * G.n4, c->eta, c->gamq, c->t_dmt, s,
                       nullptr, nullptr, nullptr, true);
     if (nfailed > 0) adga::launch_zero_points(nb, (long)g.n2 * g.N, c->keep, c->t_dmt, s);  // keep flags on the device
-    // d Sigma_nl = sum_q [d m_tot . G(k - q) + m_tot . dG(k - q)]
-    z64* sig = c->t_sig + dir * nsig;
-    adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->t_dmt, c->gt, sig, s);
-    adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, dgt, sig, s);
+    // d Sigma_nl = sum_q [d m_tot . G(k - q) + m_tot . dG(k - q)]: with the EOM as k-grid convolutions,
+    // d m_tot is accumulated on the q grid beside m_tot and both terms are formed when the omega's EOM is
+    // flushed; else as direct sums per batch
+    if (c->eom_mode == 1 && c->t_macc) {
+      adga::launch_eom_scatter(g, nb, c->qidx, c->t_dmt, c->t_macc + dir * (size_t)G.iwf * G.n4 * G.nkp, s);
+    } else {
+      z64* sig = c->t_sig + dir * nsig;
+      adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->t_dmt, c->gt, sig, s);
+      adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->mt, dgt, sig, s);
+    }
   }
   CK(hipGetLastError());
   // outputs (ndim^2, ndim^2, npts, ndir)
@@ -152,6 +182,21 @@ int adga_set_tangents(adga_ctx* c, int ndir, const double* dhk, const double* ds
     const size_t nsig = (size_t)g.n2 * g.nf * g.nkp_eom * ndir;
     dalloc(&c->t_sig, nsig);
     CK(hipMemsetAsync(c->t_sig, 0, sizeof(z64) * nsig, s));
+    const bool eom_dft = c->do_eom && c->eom_mode == 1;
+    if (eom_dft || c->bubble_mode != 0) {
+      // the derivatives by k-grid convolution (EOM, bubble): each direction's dG table transformed once
+      conv_tables(c);
+      const size_t ngt = (size_t)g.nw * g.n2 * g.nkp;
+      dalloc(&c->t_dghat, ngt * ndir);
+      DevBuf<> tmp(ngt);
+      for (int d = 0; d < ndir; ++d) adga::launch_ghat(g, c->tw, c->dgt + d * ngt, tmp.p, c->t_dghat + d * ngt, s);
+    }
+    if (eom_dft) {
+      const size_t nm = (size_t)g.iwf * g.n4 * g.nkp;
+      dalloc(&c->t_macc, nm * ndir);
+      CK(hipMemsetAsync(c->t_macc, 0, sizeof(z64) * nm * ndir, s));
+      dalloc(&c->t_mwork, nm);
+    }
     CK(hipGetLastError());
     CK(hipStreamSynchronize(s));
     return ADGA_OK;
@@ -208,6 +253,7 @@ int adga_get_sigma_tangent(adga_ctx* c, double* dsigma_nl) {
     const Geom& g = c->g;
     hipStream_t s = c->stream;
     const size_t nsig = (size_t)g.n2 * g.nf * g.nkp_eom;
+    flush_eom(c);  // the pending omega's EOM and its derivatives
     for (int dir = 0; dir < c->ndir; ++dir) adga::launch_fill_neg(g, g.nkp_eom, c->t_sig + dir * nsig, s);
     CK(hipGetLastError());
     download(dsigma_nl, c->t_sig, nsig * c->ndir, s);

@@ -104,6 +104,13 @@ struct adga_ctx {
   z64 *t_dchi0q = nullptr, *t_dS = nullptr, *t_Z1 = nullptr, *t_Z2 = nullptr, *t_B = nullptr, *t_dY = nullptr,
       *t_deta = nullptr, *t_dgamq = nullptr, *t_dmt = nullptr, *t_dchi = nullptr, *t_dbub = nullptr;
   z64* t_sig = nullptr;
+  // EOM derivatives by k-grid convolution (eom mode 1): dm per direction on the q grid, the transform of
+  // each direction's dG table, scratch
+  z64 *t_macc = nullptr, *t_dghat = nullptr, *t_mwork = nullptr;
+  // bubble derivatives of every q of one omega (k-grid convolutions), per direction, and their scratch
+  z64 *t_dbub_all = nullptr, *t_bub_p = nullptr;
+  bool t_bub_valid = false;
+  int t_bub_iwb = 0, t_bub_iwf = 0;
   // device staging of caller arrays for the module-level entry points (api_modules.cpp)
   z64* stage[6] = {};
   size_t stage_n[6] = {};
@@ -211,5 +218,9 @@ inline int op_finish(adga_ctx* c, bool launched, const char* what) {
 void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>& keep, int nfailed,
                    const TangentOut& out);
 void free_tangent(adga_ctx* c);
+// the twiddles and the k-grid transform of the G table (formed once); the pending omega's EOM flushed
+// (with its derivatives when tangent directions are set) (api.cpp)
+void conv_tables(adga_ctx* c);
+void flush_eom(adga_ctx* c);
 
 }  // namespace adga_host

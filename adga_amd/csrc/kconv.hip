@@ -100,6 +100,32 @@ void launch_bubble_all(const Geom& g, int iwb, const z64* tw, const z64* ghat, z
   dft3(g, (long)g.nf * g.n4, -1, tw, P, out, s);
 }
 
+// the bubble's derivative by the product rule, for every q:
+// P[(d, c)][R] = Ghat(nu_d)_il(-R) dGhat(nu_d - omega)_k'j(R) + dGhat(nu_d)_il(-R) Ghat(nu_d - omega)_k'j(R)
+__global__ void k_bub_prod_tangent(Geom g, int iwb, const z64* __restrict__ ghat, const z64* __restrict__ dghat,
+                                   z64* __restrict__ P) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)g.nf * g.n4 * g.nkp;
+  if (idx >= tot) return;
+  int R = (int)(idx % g.nkp);
+  long dc = idx / g.nkp;
+  int c = (int)(dc % g.n4);
+  int d = (int)(dc / g.n4);
+  int il = c / g.n2, kj = c % g.n2;
+  int rz = R % g.nkz, ry = (R / g.nkz) % g.nky, rx = R / (g.nky * g.nkz);
+  int mR = ((g.nkz - rz) % g.nkz) + ((g.nky - ry) % g.nky) * g.nkz + ((g.nkx - rx) % g.nkx) * g.nky * g.nkz;
+  int iwf = d - g.iwf;
+  const long oa = ((long)(iwf - g.glo) * g.n2 + il) * g.nkp, ob = ((long)(iwf - iwb - g.glo) * g.n2 + kj) * g.nkp;
+  P[idx] = cadd(cmul(ghat[oa + mR], dghat[ob + R]), cmul(dghat[oa + mR], ghat[ob + R]));
+}
+
+void launch_bubble_all_tangent(const Geom& g, int iwb, const z64* tw, const z64* ghat, const z64* dghat, z64* P,
+                               z64* out, hipStream_t s) {
+  long tot = (long)g.nf * g.n4 * g.nkp;
+  hipLaunchKernelGGL(k_bub_prod_tangent, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, iwb, ghat, dghat, P);
+  dft3(g, (long)g.nf * g.n4, -1, tw, P, out, s);
+}
+
 // chi0q[b][d](i1 = i*n + j, i2 = l*n + k') = -beta/nkp^2 all[(d, c)][q_b]
 // (the -beta/nkp of accumulate_chi0 and main.F90:653, and the 1/nkp of the convolution theorem)
 __global__ void k_bub_gather(Geom g, int npts, const int* __restrict__ qidx, const z64* __restrict__ all,
@@ -149,7 +175,7 @@ void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt,
 
 // H[(nu, i, l)][R] = sum_{j,k'} Mhat[(nu, i*n + j, l, k')][R] Ghat(nu - omega)_jk'(R)
 __global__ void k_eom_contract(Geom g, int iwb, const z64* __restrict__ mhat, const z64* __restrict__ ghat,
-                               z64* __restrict__ H) {
+                               z64* __restrict__ H, int accumulate) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long tot = (long)g.iwf * g.n2 * g.nkp;
   if (idx >= tot) return;
@@ -165,7 +191,7 @@ __global__ void k_eom_contract(Geom g, int iwb, const z64* __restrict__ mhat, co
       long mf = (((long)nu * g.n2 + i * g.n + j) * g.n + l) * g.n + kp;
       acc = cfma(mhat[mf * g.nkp + R], gw[(long)(j * g.n + kp) * g.nkp + R], acc);
     }
-  H[idx] = acc;
+  H[idx] = accumulate ? cadd(H[idx], acc) : acc;
 }
 
 // Sigma_nl(i, l, iwf + nu, kk) += 1/nkp Hk[(nu, i, l)][k_eom(kk)]
@@ -185,14 +211,25 @@ __global__ void k_eom_add(Geom g, const int* __restrict__ keom, const z64* __res
 
 // Flush of one omega: transform the accumulated m_tot, contract with Ghat, transform back, add.
 void launch_eom_flush(const Geom& g, int iwb, const int* keom, const z64* tw, const z64* ghat, z64* macc, z64* work,
-                      z64* sigma, hipStream_t s) {
+                      z64* sigma, hipStream_t s, const EomTangent* tan) {
   long nfield = (long)g.iwf * g.n4;
   dft3(g, nfield, -1, tw, macc, work, s);  // Mhat in work (macc is scratch afterwards)
   long th = (long)g.iwf * g.n2 * g.nkp;
-  hipLaunchKernelGGL(k_eom_contract, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, g, iwb, work, ghat, macc);
-  dft3(g, (long)g.iwf * g.n2, +1, tw, macc, work, s);  // H(k) in work
   long ta = (long)g.n2 * g.iwf * g.nkp_eom;
-  hipLaunchKernelGGL(k_eom_add, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, g, keom, work, sigma);
+  const dim3 gth((unsigned)((th + 255) / 256)), gta((unsigned)((ta + 255) / 256));
+  // derivatives along each tangent direction d: d Sigma = conv(dm_d, G) + conv(m, dG_d), i.e. in real
+  // space dMhat_d Ghat + Mhat dGhat_d (each direction's accumulator is scratch afterwards)
+  for (int d = 0; tan && d < tan->ndir; ++d) {
+    z64* dm = tan->macc + d * tan->nm;
+    dft3(g, nfield, -1, tw, dm, tan->work, s);  // dMhat_d in tan->work
+    hipLaunchKernelGGL(k_eom_contract, gth, dim3(256), 0, s, g, iwb, tan->work, ghat, dm, 0);
+    hipLaunchKernelGGL(k_eom_contract, gth, dim3(256), 0, s, g, iwb, work, tan->dghat + d * tan->ng, dm, 1);
+    dft3(g, (long)g.iwf * g.n2, +1, tw, dm, tan->work, s);
+    hipLaunchKernelGGL(k_eom_add, gta, dim3(256), 0, s, g, keom, tan->work, tan->sigma + d * tan->nsig);
+  }
+  hipLaunchKernelGGL(k_eom_contract, gth, dim3(256), 0, s, g, iwb, work, ghat, macc, 0);
+  dft3(g, (long)g.iwf * g.n2, +1, tw, macc, work, s);  // H(k) in work
+  hipLaunchKernelGGL(k_eom_add, gta, dim3(256), 0, s, g, keom, work, sigma);
 }
 
 }  // namespace adga

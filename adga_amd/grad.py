@@ -68,12 +68,13 @@ def forward_arrays(wl, points=None, batch=None):
     return _assemble(out, inp, points)
 
 
-def tangent_arrays(wl, points=None, batch=None):
-    """d(chi_m(q, omega = 0))/d theta and d Sigma_DGA / d theta, the last axis over (t, t', U)."""
+def tangent_arrays(wl, points=None, batch=None, eom=None, bubble=None):
+    """d(chi_m(q, omega = 0))/d theta and d Sigma_DGA / d theta, the last axis over (t, t', U); `eom` and
+    `bubble` the context's modes (0 direct sums, 1 k-grid convolutions; defaults 1 and 2, see Context)."""
     inp = synth.build_inputs(wl)
     tan = synth.tangents(inp)
     points = _points(wl) if points is None else points
-    with api.Context(inp, batch=batch) as ctx:
+    with api.Context(inp, batch=batch, eom=eom, bubble=bubble) as ctx:
         ctx.set_tangents(tan)
         out = api.run_loop_tangent(ctx, points, lambda w, c: synth.chi_loc(inp, w, c),
                                    lambda w, c: synth.dchi_loc(inp, tan, w))

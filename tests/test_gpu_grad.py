@@ -42,12 +42,16 @@ def _oracle_outputs(wl, points):
     return fin["chi_m"][:, :, sel], fin["sigma_dga"]
 
 
-def test_gpu_tangents_match_oracle_differences():
+@pytest.mark.parametrize("modes", [(1, 1), (0, 0)], ids=["dft", "direct"])
+def test_gpu_tangents_match_oracle_differences(modes):
+    """Both forms of the bubble and EOM derivatives: k-grid convolutions (the bubble's derivative for every
+    q of an omega; dm accumulated on the q grid and convolved with G and dG when the omega is flushed), or
+    direct sums per batch."""
     from adga_amd import grad
     wl = synth.workload("t2g_k4")
     points = [(w, q) for w in (-1, 0, 1) for q in range(0, 64, 4)]
-    print("GPU tangent pass, t2g_k4", flush=True)
-    dchi, dsig = grad.tangent_arrays(wl, points)
+    print(f"GPU tangent pass, t2g_k4, bubble/eom modes {modes}", flush=True)
+    dchi, dsig = grad.tangent_arrays(wl, points, bubble=modes[0], eom=modes[1])
     theta = np.array([wl.t, wl.tp, wl.U])
     for p, name in enumerate(synth.PARAMS):
         print(f"oracle central difference along {name}", flush=True)
