@@ -78,3 +78,13 @@ def test_gpu_shrinking_box_five_orbitals(precision):
     box_fn = lambda w: 80 - 2 * abs(w)  # noqa: E731
     points = [(0, 0), (5, 1234), (-20, 77)]
     _run("d5_k16", points, box_fn, precision, 1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "refined"])
+def test_gpu_shrinking_box_with_large_bubble_range(precision):
+    """The box shrinking with |omega| together with the bubble over the large frequency range (the debug
+    option "Bubble", bubble_range = 1): each omega's outer frequencies are the large range outside that
+    omega's own box; every slice against the oracle run with the slice's box and the same range."""
+    box_fn = lambda w: 6 - abs(w)  # noqa: E731
+    points = [(w, q) for w in (0, 1, -2) for q in range(0, 64, 5)]
+    _run("t2g_k4", points, box_fn, precision, 1e-10, bubble_range=1)
