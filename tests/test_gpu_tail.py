@@ -70,3 +70,26 @@ def test_gpu_bubble_range_change_needs_the_local_data_again():
         _, _, big = ctx.set_local_omega(1, xd, xm)
         assert np.abs(big - box).max() > 1e-6  # the large range adds frequencies
         assert lib.adga_set_option(ctx.h, b"bubble_range", 3.0) != 0
+
+
+@pytest.mark.timeout(600)
+def test_gpu_tail_full_size_sampled():
+    """The benchmark workload (3-orbital t2g, 20^3 k, 2x60 x 31, N = 1080, iwmax = 300: 450 outer
+    frequencies) with bubble_range = 2, two points at two omegas against the oracle, both precisions."""
+    import adga_oracle
+    from adga_amd import api
+    wl = synth.workload("t2g_k20", bubble_range=2)
+    inp = synth.build_inputs(wl)
+    pts = [(0, 0), (3, 4321)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    print("oracle t2g_k20 bubble_range=2, 2 points", flush=True)
+    ref = adga_oracle.HotPath(inp).run(pts, cl)
+    for precision in ("fp64", "refined"):
+        print(f"GPU {precision}", flush=True)
+        with api.Context(inp) as ctx:
+            out = api.run_loop(ctx, pts, cl, precision)
+        assert (out["status"] == 0).all()
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+            assert rel_err(out[k], ref[k]) < 1e-10, (precision, k, rel_err(out[k], ref[k]))
+        for w in ref["bubble_loc"]:
+            assert rel_err(out["bubble_loc"][w], ref["bubble_loc"][w]) < 1e-12, w
