@@ -39,6 +39,7 @@ struct LamError {
   hipError_t e;
   const char* what;
 };
+struct NonFinite {};  // a sum of the susceptibilities is not finite
 
 #define LCK(x)                                    \
   do {                                            \
@@ -53,6 +54,9 @@ int lam_guarded(adga_lambda* h, F&& f) {
   } catch (const LamError& le) {
     if (h) h->err = std::string("HIP error in ") + le.what + ": " + hipGetErrorString(le.e);
     return le.e == hipErrorOutOfMemory ? ADGA_E_NOMEM : ADGA_E_HIP;
+  } catch (const NonFinite&) {
+    if (h) h->err = "the susceptibilities' sum is not finite";
+    return ADGA_E_NONFINITE;
   } catch (const std::exception& ex) {
     if (h) h->err = std::string("host exception: ") + ex.what();
     return ADGA_E_HIP;
@@ -183,12 +187,13 @@ int adga_lambda_target(adga_lambda* h, double* target) {
 }
 
 int adga_lambda_solve(adga_lambda* h, double target, double tol, int maxit, double* lambda, int* iters) {
-  if (!h || !lambda || !(tol > 0) || maxit < 1) return ADGA_E_ARG;
+  if (!h || !lambda || !(tol > 0) || maxit < 1 || !std::isfinite(target)) return ADGA_E_ARG;
   return lam_guarded(h, [&]() -> int {
     const double scale = 1.0 / (h->beta * h->nqp);
     auto f = [&](double lam, double* df) {
       z64 res[2];
       sums(h, h->chi, h->npts, scale, 1, &lam, res);
+      if (!std::isfinite(res[0].re) || !std::isfinite(res[1].re)) throw NonFinite{};
       *df = res[1].re;
       return res[0].re - target;
     };

@@ -373,7 +373,8 @@ int adga_lambda_target(adga_lambda *h, double *target);
 /*
  * lambda in (-1/max Re chi_phys(q, omega = 0), inf) with |Re S(lambda) - target| <= tol max(1, |target|)
  * (Newton's method on the device sums, bisection where a step leaves the bracket); ADGA_E_NOCONV after
- * maxit steps.  iters (may be NULL): the steps taken.
+ * maxit steps, ADGA_E_NONFINITE when a sum is not finite, ADGA_E_ARG for a non-finite target.  iters
+ * (may be NULL): the steps taken.
  */
 int adga_lambda_solve(adga_lambda *h, double target, double tol, int maxit, double *lambda, int *iters);
 /* chi_phys^lambda of every point, (nqp*(2*iwbmax+1)) complex */

@@ -86,3 +86,19 @@ def test_gpu_lambda_without_positive_static_susceptibility_is_refused():
     with pytest.raises(api.AdgaError) as e:
         api.Lambda(chi, loc, 5.0, 4)
     assert e.value.code == 1  # ADGA_E_ARG
+
+
+def test_gpu_lambda_non_finite_input_is_reported():
+    """A NaN target is refused; a non-finite susceptibility makes the solve report ADGA_E_NONFINITE
+    instead of returning a lambda."""
+    from adga_amd import api
+    beta, nqp = 7.0, 10
+    chi = np.full(3 * nqp, 0.8)
+    chi[4] = np.nan  # an omega != 0 point: the pole is still bounded by the omega = 0 points
+    with api.Lambda(_as_chi_qw(chi), _as_chi_qw(np.full(3, 0.8)), beta, nqp) as lam:
+        with pytest.raises(api.AdgaError) as e:
+            lam.solve(target=float("nan"))
+        assert e.value.code == 1  # ADGA_E_ARG
+        with pytest.raises(api.AdgaError) as e:
+            lam.solve(target=0.1)
+        assert e.value.code == 5  # ADGA_E_NONFINITE
