@@ -6,7 +6,7 @@ tangent-linear mode of the hot path) and the torch.autograd.Function around them
   bosonic frequencies).
 * At the configuration of the differentiation workload (3-orbital t2g, 12^3 k, 2x40 fermionic x 21 bosonic,
   N = 720, every point): torch's gradient of a scalar loss against four-point central differences of the GPU
-  forward pass.
+  forward pass, and against two-point central differences at step 1e-5.
 The derivatives are small next to the values (d Sigma / dt ~ 5e-5 with |Sigma| ~ 14), so a two-point difference
 at a step small enough for its h^2 error carries a round-off of eps |Sigma| / (h |dSigma|) ~ 1e-6; the
 four-point stencil [8 (f(h) - f(-h)) - (f(2h) - f(-2h))] / 12h at h = 1e-3 has an h^4 error below 1e-8 and a
@@ -63,7 +63,8 @@ def test_gpu_tangents_match_oracle_differences(modes):
 
 @pytest.mark.timeout(600)
 def test_gpu_autograd_t2g_k12_against_central_differences():
-    """About a minute: 14 passes over the 36288 points (forward, tangent, 12 for the differences)."""
+    """About a minute and a half: 20 passes over the 36288 points (forward, tangent, 18 for the
+    differences)."""
     import torch
     from adga_amd import grad
     wl = synth.workload("t2g_k12")
@@ -84,8 +85,13 @@ def test_gpu_autograd_t2g_k12_against_central_differences():
     th = theta.detach().numpy()
     for p, name in enumerate(synth.PARAMS):
         print(f"central difference along {name}", flush=True)
-        fd, = _four_point(lambda t: (np.array(loss(*[torch.from_numpy(a) for a in
-                                                      grad.forward_arrays(grad.with_params(wl, t))]).item()),),
-                          th, p)
-        print(f"  dL/d{name}: autograd {g[p]:.12e}, central difference {fd:.12e}", flush=True)
+        fl = lambda t: loss(*[torch.from_numpy(a) for a in grad.forward_arrays(grad.with_params(wl, t))]).item()  # noqa: E731
+        fd, = _four_point(lambda t: (np.array(fl(t)),), th, p)
+        # the two-point difference at step 1e-5 as well: for the scalar loss (|L| ~ 1e2) its round-off,
+        # eps |L| / h ~ 2e-9, and its h^2 error stay far below the bound
+        e = np.zeros(3)
+        e[p] = 1e-5
+        fd2 = (fl(th + e) - fl(th - e)) / 2e-5
+        print(f"  dL/d{name}: autograd {g[p]:.12e}, four-point {fd:.12e}, two-point (1e-5) {fd2:.12e}", flush=True)
         assert abs(g[p] - fd) <= 1e-6 * max(abs(fd), 1e-3 * abs(L.item())), name
+        assert abs(g[p] - fd2) <= 1e-6 * max(abs(fd2), 1e-3 * abs(L.item())), name
