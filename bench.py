@@ -189,9 +189,10 @@ def one_particle(api, synth, iwmax=2000, reps=3, cpu_iwmax=200, cpu=True):
 
 
 def level1_inverse(api, N, reps=3):
-    """The one-matrix drop-in for inverse_matrix_z (adga_inverse_matrix_z; INTEGRATION.md level 1): best-of-
-    `reps` wall time of one N x N complex128 inverse through the C ABI, host copies included, and the
-    (q, omega) rate it would cap the loop at (two inverses per point, everything else free)."""
+    """The one-matrix drop-in for inverse_matrix_z (adga_inverse_matrix_z, which the replacement
+    lapack_module calls with the reference's main.F90 unchanged): best-of-`reps` wall time of one N x N
+    complex128 inverse through the C ABI, host copies included, and the (q, omega) rate it would cap the
+    loop at (two inverses per point, everything else free)."""
     rng = np.random.default_rng(1)
     a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)) + 4 * np.sqrt(N) * np.eye(N)
     api.inverse_matrix(a)
