@@ -11,6 +11,10 @@ module adga_c
             c_adga_inverse_matrix_z, c_adga_get_giw, c_adga_get_nfock, c_adga_get_sigma_g_loc, c_adga_get_ndga, &
             c_adga_set_vq, c_adga_bubble_q, c_adga_chi0_loc, c_adga_chi_qw, c_adga_eom_dynamic, c_adga_eom_dmft, &
             c_adga_eom_static, c_adga_get_gkiw, c_adga_get_ndmft, c_adga_add_siw_dmft, &
+            c_adga_set_local_omega_box, c_adga_inverse_matrix_z_noalloc, c_adga_set_tangents, &
+            c_adga_set_local_omega_tangent, c_adga_run_points_tangent, c_adga_get_sigma_tangent, &
+            c_adga_lambda_create, c_adga_lambda_sums, c_adga_lambda_target, c_adga_lambda_solve, c_adga_lambda_chi, &
+            c_adga_lambda_last_error, c_adga_lambda_destroy, &
             ADGA_OK, ADGA_E_SINGULAR, ADGA_PREC_FP64, ADGA_PREC_REFINED
 
   integer(c_int), parameter :: ADGA_OK = 0, ADGA_E_SINGULAR = 3
@@ -190,5 +194,108 @@ module adga_c
        type(c_ptr), value :: ctx
        complex(c_double_complex), intent(inout) :: sigma_sum(*)
      end function c_adga_add_siw_dmft
+
+     integer(c_int) function c_adga_set_local_omega_box(ctx, iwb, iwf_box, chi_loc_dens, chi_loc_magn, &
+          chi_loc_dens_out, chi_loc_magn_out, bubble_loc_out) bind(c, name='adga_set_local_omega_box')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iwb, iwf_box
+       complex(c_double_complex), intent(in) :: chi_loc_dens(*), chi_loc_magn(*)
+       complex(c_double_complex), intent(out) :: chi_loc_dens_out(*), chi_loc_magn_out(*), bubble_loc_out(*)
+     end function c_adga_set_local_omega_box
+
+     integer(c_int) function c_adga_inverse_matrix_z_noalloc(n, m, ierr) bind(c, name='adga_inverse_matrix_z_noalloc')
+       import :: c_int, c_double_complex
+       integer(c_int), value :: n
+       complex(c_double_complex), intent(inout) :: m(*)
+       integer(c_int), intent(out) :: ierr
+     end function c_adga_inverse_matrix_z_noalloc
+
+     ! derivatives along ndir directions of the inputs (no counterpart in the reference)
+     integer(c_int) function c_adga_set_tangents(ctx, ndir, dhk, dsiw, dgiw, dmu, du, du_tilde) &
+          bind(c, name='adga_set_tangents')
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndir
+       complex(c_double_complex), intent(in) :: dhk(*), dsiw(*), dgiw(*), du(*), du_tilde(*)
+       real(c_double), intent(in) :: dmu(*)
+     end function c_adga_set_tangents
+
+     integer(c_int) function c_adga_set_local_omega_tangent(ctx, dchi_loc_dens, dchi_loc_magn, dchi_loc_dens_out, &
+          dchi_loc_magn_out, dbubble_loc_out) bind(c, name='adga_set_local_omega_tangent')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: dchi_loc_dens(*), dchi_loc_magn(*)
+       complex(c_double_complex), intent(out) :: dchi_loc_dens_out(*), dchi_loc_magn_out(*), dbubble_loc_out(*)
+     end function c_adga_set_local_omega_tangent
+
+     integer(c_int) function c_adga_run_points_tangent(ctx, npts, iq, chi_dens_out, chi_magn_out, bubble_nl_out, &
+          dchi_dens_out, dchi_magn_out, dbubble_nl_out, status) bind(c, name='adga_run_points_tangent')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: npts
+       integer(c_int), intent(in) :: iq(*)
+       complex(c_double_complex), intent(out) :: chi_dens_out(*), chi_magn_out(*), bubble_nl_out(*)
+       complex(c_double_complex), intent(out) :: dchi_dens_out(*), dchi_magn_out(*), dbubble_nl_out(*)
+       integer(c_int), intent(out) :: status(*)
+     end function c_adga_run_points_tangent
+
+     integer(c_int) function c_adga_get_sigma_tangent(ctx, dsigma_nl) bind(c, name='adga_get_sigma_tangent')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: dsigma_nl(*)
+     end function c_adga_get_sigma_tangent
+
+     ! lambda correction of the final susceptibility (no counterpart in the reference)
+     integer(c_int) function c_adga_lambda_create(ndim, nqp, iwbmax, beta, chi_qw, chi_loc, h) &
+          bind(c, name='adga_lambda_create')
+       import :: c_int, c_ptr, c_double, c_double_complex
+       integer(c_int), value :: ndim, nqp, iwbmax
+       real(c_double), value :: beta
+       complex(c_double_complex), intent(in) :: chi_qw(*), chi_loc(*)
+       type(c_ptr), intent(out) :: h
+     end function c_adga_lambda_create
+
+     integer(c_int) function c_adga_lambda_sums(h, nl, lambdas, s, ds) bind(c, name='adga_lambda_sums')
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nl
+       real(c_double), intent(in) :: lambdas(*)
+       complex(c_double_complex), intent(out) :: s(*), ds(*)
+     end function c_adga_lambda_sums
+
+     integer(c_int) function c_adga_lambda_target(h, target) bind(c, name='adga_lambda_target')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: target
+     end function c_adga_lambda_target
+
+     integer(c_int) function c_adga_lambda_solve(h, target, tol, maxit, lambda, iters) bind(c, name='adga_lambda_solve')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: target, tol
+       integer(c_int), value :: maxit
+       real(c_double), intent(out) :: lambda
+       integer(c_int), intent(out) :: iters
+     end function c_adga_lambda_solve
+
+     integer(c_int) function c_adga_lambda_chi(h, lambda, chi_phys) bind(c, name='adga_lambda_chi')
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: h
+       real(c_double), value :: lambda
+       complex(c_double_complex), intent(out) :: chi_phys(*)
+     end function c_adga_lambda_chi
+
+     integer(c_int) function c_adga_lambda_last_error(h, buf, len) bind(c, name='adga_lambda_last_error')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int), value :: len
+     end function c_adga_lambda_last_error
+
+     integer(c_int) function c_adga_lambda_destroy(h) bind(c, name='adga_lambda_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function c_adga_lambda_destroy
   end interface
 end module adga_c

@@ -255,8 +255,10 @@ int adga_set_local_omega_tangent(adga_ctx *ctx, const double *dchi_loc_dens, con
 /*
  * adga_run_points in complex128 that also returns, per direction, the derivatives of chi_dens, chi_magn and
  * bubble_nl (ndim^2, ndim^2, npts, ndir; may be NULL) and accumulates the derivative of Sigma_nl on the
- * device.  Each direction reuses the batch's LU factors: a right-hand side and a solve, the bubble formed
- * again with one factor differentiated at a time, and the EOM twice.
+ * device.  Each direction reuses the batch's LU factors: a right-hand side and a solve; the bubble's and
+ * the EOM's derivatives follow the context's "bubble" and "eom" options (k-grid convolutions: the bubble's
+ * derivative for every q of the omega, dm accumulated on the q grid and convolved when the omega's EOM is
+ * flushed; direct sums: per batch).
  */
 int adga_run_points_tangent(adga_ctx *ctx, int npts, const int *iq, double *chi_dens_out, double *chi_magn_out,
                             double *bubble_nl_out, double *dchi_dens_out, double *dchi_magn_out,

@@ -28,6 +28,14 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == _declared()
 
 
+def test_fortran_layer_binds_every_declared_entry_point():
+    """adga_amd/fortran/adga_c.f90 has a bind(c) interface for each entry point of the header, so a
+    Fortran host reaches all of them."""
+    src = open(os.path.join(ROOT, "adga_amd", "fortran", "adga_c.f90")).read()
+    bound = sorted(set(re.findall(r"bind\(c,\s*name='(adga_\w+)'\)", src)))
+    assert bound == _declared()
+
+
 def test_argument_validation_without_gpu():
     from adga_amd import _lib
     lib = _lib.lib
