@@ -2,6 +2,7 @@
 // refinement loop and error reporting.  No exception crosses the extern "C" boundary.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "ctx.h"
 
@@ -147,6 +148,58 @@ const std::vector<int>& ensure_ops(adga_ctx* c, int prec) {
 bool carry_rhs(const adga_ctx* c, int prec) {
   const Geom& g = c->gw;
   return !adga::lu_solve_one_group(g.N, g.n2, prec == 0 ? sizeof(z64) : sizeof(c32)) && c->ldm >= g.N + g.n2;
+}
+
+// The lane tuning table of the process, per (device, batch shape).  The fastest lane count is a property
+// of the device and the shape, so every context shares it; contexts may run on several host threads.
+std::mutex g_lane_mu;
+std::map<std::pair<int, long>, LaneTune> g_lane_tune;
+
+void lane_result(LaneTune& lt, int lanes, float ms) {
+  const int bit = 1 << (lanes - 1);
+  if (lt.done & bit) return;
+  lt.ms[lanes - 1] = ms;
+  lt.done |= bit;
+  if (lt.done == (1 << LANE_CANDIDATES) - 1)
+    lt.best = 1 + (int)(std::min_element(lt.ms, lt.ms + LANE_CANDIDATES) - lt.ms);
+}
+
+// lane count of the next factorisation of nmat matrices of size N.  A fixed option value is used as is.
+// With the tuned default the best lane count depends on the batch shape (measured: 2 lanes fastest for
+// 4096 matrices of N = 1080, 1 lane for 256 of N = 2000 .. 4000, 3 for 8000 of N = 1080, by 0.5-2%), so
+// a shape's first factorisations on the device each take one untried candidate; the context keeps the
+// trial in lane_key / lane_trial and lane_store files its time when the batch is done.  A trial left
+// pending by a failed batch counts as the slowest.
+int pick_lanes(adga_ctx* c, int prec, int N, int nmat) {
+  std::lock_guard<std::mutex> lk(g_lane_mu);
+  if (c->lane_key >= 0) lane_result(g_lane_tune[{c->device, c->lane_key}], c->lane_trial, INFINITY);
+  c->lane_key = -1;
+  if (c->lu_split > 0) return c->lu_split;
+  const long key = ((long)prec << 44) | ((long)N << 22) | (long)nmat;  // nmat <= 2^21, N <= 4096
+  LaneTune& lt = g_lane_tune[{c->device, key}];
+  if (lt.done == (1 << LANE_CANDIDATES) - 1) return lt.best;
+  if (lt.issued < LANE_CANDIDATES) {
+    c->lane_key = key;
+    c->lane_trial = ++lt.issued;
+    return c->lane_trial;
+  }
+  // every candidate is out, some still running in another context: the fastest timed so far
+  int best = 1;
+  float bm = INFINITY;
+  for (int j = 0; j < LANE_CANDIDATES; ++j)
+    if ((lt.done >> j & 1) && lt.ms[j] < bm) {
+      bm = lt.ms[j];
+      best = j + 1;
+    }
+  return best;
+}
+
+// files the time of the context's pending trial
+void lane_store(adga_ctx* c, float ms) {
+  if (c->lane_key < 0) return;
+  std::lock_guard<std::mutex> lk(g_lane_mu);
+  lane_result(g_lane_tune[{c->device, c->lane_key}], c->lane_trial, ms);
+  c->lane_key = -1;
 }
 
 const std::vector<int>& ensure_ops_rhs(adga_ctx* c, int prec) {
@@ -330,6 +383,7 @@ int adga_ctx_create(const adga_dims* d, const int* q_data, const int* k_data_eom
     }
     for (int i = 0; i < g.nkp_eom; ++i)
       if (k_data_eom[i] < 1 || k_data_eom[i] > g.nkp) throw HipError{hipErrorInvalidValue, "k_data_eom out of range"};
+    CK(hipGetDevice(&c->device));
     CK(hipStreamCreate(&c->stream));
     c->lanes.s[0] = c->stream;
     for (int j = 1; j < adga::LU_MAX_LANES; ++j) CK(hipStreamCreate(&c->lanes.s[j]));
@@ -396,7 +450,8 @@ int adga_set_option(adga_ctx* c, const char* key, double v) {
     return ADGA_OK;
   }
   if (!strcmp(key, "lu_streams")) {
-    if (!(v >= 1 && v <= adga::LU_MAX_LANES && v == (int)v)) return fail(c, ADGA_E_ARG, "lu_streams must be 1 .. 4");
+    if (!(v >= 0 && v <= adga::LU_MAX_LANES && v == (int)v))
+      return fail(c, ADGA_E_ARG, "lu_streams must be 0 (tuned per batch shape) or 1 .. 4");
     c->lu_split = (int)v;
     return ADGA_OK;
   }
@@ -590,7 +645,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       const std::vector<int>& ops = rhs ? ensure_ops_rhs(c, prec) : ensure_ops(c, prec);
       const int W = rhs ? g.N + n2 : g.N;
       adga::LuLanes lanes = c->lanes;
-      lanes.n = c->lu_split;
+      lanes.n = pick_lanes(c, prec, g.N, nmat);
       relh.assign(nmat, 0.0);
       infoh.assign(nmat, 0);
       resolved.assign(nmat, 0);
@@ -713,6 +768,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       c->stats[2 + 5] += ms[5];            // EOM
       float lums = 0.f;
       CK(hipEventElapsedTime(&lums, c->ev_lu[0], c->ev_lu[1]));
+      lane_store(c, lums);
       c->stats[9] += lums;                 // the batched LU kernel alone
       c->stats[10] += 1;                   // its launches
       c->stats[11] += c->lut.collect();    // the dominant GEMM kernel: milliseconds,
@@ -877,6 +933,7 @@ int adga_last_error(adga_ctx* c, char* buf, int len) {
 
 int adga_ctx_destroy(adga_ctx* c) {
   if (!c) return ADGA_OK;
+  lane_store(c, INFINITY);  // a lane trial left pending by a failed batch counts as the slowest
   free_batch(c);
   free_fallback(c);
   dfree(c->hk); dfree(c->siw); dfree(c->giw); dfree(c->dc); dfree(c->u); dfree(c->ut); dfree(c->ndmft);

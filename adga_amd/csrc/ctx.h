@@ -20,6 +20,15 @@ using adga::z64;
 
 constexpr int ADGA_NSTATS = 16;  // values adga_get_stats reports
 
+// LU lane counts tried per batch shape, and the factorisation times (ms) each gave
+constexpr int LANE_CANDIDATES = 3;
+struct LaneTune {
+  float ms[LANE_CANDIDATES] = {};
+  int issued = 0;  // candidates handed out for a timed trial so far (lane counts 1 .. issued)
+  int done = 0;    // bit j: the time of lane count j + 1 is in ms[j]
+  int best = 1;    // the fastest, once every candidate is timed
+};
+
 // Buffers of the FP64 re-solve of points whose complex64 refinement did not converge: the points'
 // S and right-hand sides are gathered into these, solved in complex128 and the solutions scattered back.
 struct FallbackBufs {
@@ -36,7 +45,13 @@ struct adga_ctx {
   Geom eom_g{};  // geometry of the omega whose EOM is pending
   hipStream_t stream = nullptr;
   adga::LuLanes lanes;                     // streams 1.. of the LU lanes (default priority) and their events
-  int lu_split = 2;                        // option "lu_streams": parts each LU batch is split into
+  int lu_split = 0;                        // option "lu_streams": parts each LU batch is split into (0: tuned)
+  // lu_streams = 0: per device and batch shape (precision, BSE size, matrices) the first factorisations
+  // in the process run with 1 .. LANE_CANDIDATES lanes, each timed by ev_lu, and the later ones with the
+  // fastest; every lane count runs the same kernels on the same data, so results do not depend on it
+  int device = 0;      // the device current at creation: the context's streams belong to it
+  long lane_key = -1;  // shape of this context's factorisation whose trial time is pending (-1: none)
+  int lane_trial = 0;  // its lane count
   hipEvent_t ev[7] = {}, ev_lu[2] = {};    // stage and LU timing of adga_run_points
   std::string err;
   int refine_maxit = 12;

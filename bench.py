@@ -205,11 +205,11 @@ def level1_inverse(api, N, reps=3):
             "note": "one matrix per call (one workgroup per panel), hipMalloc/hipFree and a host transpose per call"}
 
 
-def shrinking_box(api, synth, batch=128, omegas=(0, 5, 20), warmup=1, steps=1):
+def shrinking_box(api, synth, batch=128, omegas=(0, 5, 20), warmup=3, steps=1):
     """The 5-orbital d-shell workload (16^3 k, 41 bosonic frequencies) with the fermionic box shrinking
     with |omega|, 2 x (80 - 2|omega|): one context with the largest box, each omega at its own BSE size
-    (adga_set_local_omega_box).  `steps` batches of `batch` q points per omega after `warmup` untimed ones,
-    refined and FP64; the full run (4096 q at each of the 41 omegas) is estimated from the measured time per
+    (adga_set_local_omega_box).  `steps` batches of `batch` q points per omega after `warmup` untimed ones
+    (in which the library tunes its LU stream count for the omega's size), refined and FP64; the full run (4096 q at each of the 41 omegas) is estimated from the measured time per
     point, fitted as a + b N^3 over the three sizes."""
     wl = synth.workload("d5_k16")
     inp = synth.build_inputs(wl)
@@ -287,15 +287,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=2048)
+    # three warm-up steps: the library times the first three factorisations of a batch shape with 1, 2 and
+    # 3 LU streams and keeps the fastest (option lu_streams 0), so the timed steps run the tuned choice
+    ap.add_argument("--warmup", type=int, default=3)
+    # 4000 points: the 8000 q points of an omega in two full batches
+    ap.add_argument("--batch", type=int, default=4000)
     ap.add_argument("--workload", default="t2g_k20")
     ap.add_argument("--cpu-points", type=int, default=6)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
                     help="single-threaded reference processes of the CPU baseline (one per core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--lu-streams", type=int, default=None, help="the library's lu_streams option (1 or 2)")
+    ap.add_argument("--lu-streams", type=int, default=None,
+                    help="the library's lu_streams option (0: tuned per batch shape, the default; 1 .. 4)")
     ap.add_argument("--no-one-particle", action="store_true", help="skip the one-particle routines' timing")
     ap.add_argument("--only-one-particle", action="store_true",
                     help="time only the one-particle routines (get_giw, get_nfock, get_sigma_g_loc, get_ndga) "

@@ -80,8 +80,10 @@ int adga_ctx_create(const adga_dims *dims, const int *q_data, const int *k_data_
  * default), "eom" (0: direct sum per batch; 1: m_tot accumulated on the q grid and convolved with
  * G by k-grid DFT when omega changes or the self-energy is read, the default), "lu_streams" (1: each
  * batch's LU on one stream; 2 .. 4: the batch split into that many parts on as many streams, each one
- * operation behind the previous, the large Schur updates on the whole batch; 2 is the default; results
- * are identical),
+ * operation behind the previous, the large Schur updates on the whole batch; 0, the default: chosen per
+ * device and batch shape (precision, BSE size, matrices in the batch): the shape's first three
+ * factorisations in the process, in any context, run with 1, 2 and 3 streams, each timed, the later
+ * ones with the fastest; results are identical for every value),
  * "bubble_range" (0: bubble_loc and bubble_nl sum chi0 over the vertex box, the default; 1: over every
  * nu for which G(nu - omega) exists, [-iwmax + iwbmax_small, iwmax - iwbmax_small), the reference's
  * debug option "Bubble" (src/config_module.f90:467-477); 2: as 1, and the bubble_loc returned by

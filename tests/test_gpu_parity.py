@@ -240,6 +240,31 @@ def test_gpu_multi_stream_lu_is_identical(t2g_full):
                 assert np.array_equal(res[1][k], res[ls][k]), (precision, ls, k)
 
 
+def test_gpu_tuned_lanes_are_identical(t2g_full):
+    """The default lane count (lu_streams 0, tuned per device and batch shape for the whole process):
+    98 points in batches of 16 make six factorisations of 32 matrices and one of 4 per pass (in a fresh
+    process the first three of 32 matrices are the timed trials with 1, 2 and 3 lanes, the rest use the
+    fastest); every result equals the one-stream run's exactly, in both modes, and an invalid option
+    value is refused."""
+    api = _api()
+    inp = t2g_full
+    pts = [(1, q) for q in range(5, 8000, 82)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    for precision in ("fp64", "refined"):
+        with api.Context(inp, batch=16, lu_streams=1) as ctx:
+            one = api.run_loop(ctx, pts, cl, precision)
+        with api.Context(inp, batch=16) as ctx:
+            tuned = api.run_loop(ctx, pts, cl, precision)
+            ctx.reset_sigma()
+            again = api.run_loop(ctx, pts, cl, precision)  # every shape tuned by now
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl"):
+            assert np.array_equal(one[k], tuned[k]), (precision, k)
+            assert np.array_equal(one[k], again[k]), (precision, k)
+    with api.Context(inp) as ctx:
+        assert api.lib.adga_set_option(ctx.h, b"lu_streams", 5.0) == 1  # ADGA_E_ARG
+        assert api.lib.adga_set_option(ctx.h, b"lu_streams", 0.0) == 0
+
+
 def test_gpu_five_orbital_large_box_against_oracle():
     """5-orbital d-shell model (random symmetric hopping), 16^3 k, 2x80 fermionic frequencies: BSE
     N = 4000, the largest size the batched LU takes (16 rows per thread in the panels; the solves
