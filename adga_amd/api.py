@@ -173,26 +173,35 @@ class Context:
         self._check(lib.adga_get_nfock(self.h, _dptr(out)))
         return out
 
-    def get_sigma_g_loc(self, sigma_sum):
+    def get_sigma_g_loc(self, sigma_sum=None):
         """(sigma_loc, gloc) like get_sigma_g_loc (src/one_particle_quant_module.f90:124-180);
-        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp) after add_siw_dmft."""
-        ss = _zf(sigma_sum)
-        if ss.shape != (self.n, self.n, self.nf, self.nkp):
-            raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp})")
+        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp) after add_siw_dmft, or None for the context's own
+        (its accumulated Sigma_nl plus siw, formed on the device)."""
+        if sigma_sum is None:
+            ptr = None
+        else:
+            ss = _zf(sigma_sum)
+            if ss.shape != (self.n, self.n, self.nf, self.nkp):
+                raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp})")
+            ptr = _dptr(ss)
         sloc = np.zeros((self.n, self.n, self.nf), dtype=np.complex128, order="F")
         gloc = np.zeros((2 * self.iwmax, self.n, self.n), dtype=np.complex128, order="F")
-        self._check(lib.adga_get_sigma_g_loc(self.h, _dptr(ss), _dptr(sloc), _dptr(gloc)))
+        self._check(lib.adga_get_sigma_g_loc(self.h, ptr, _dptr(sloc), _dptr(gloc)))
         return sloc, gloc
 
-    def get_ndga(self, sigma_sum):
+    def get_ndga(self, sigma_sum=None):
         """(n_dga_k, n_dga) like get_ndga (src/one_particle_quant_module.f90:363-393);
-        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) after add_siw_dmft."""
-        ss = _zf(sigma_sum)
-        if ss.shape != (self.n, self.n, self.nf, self.nkp_eom):
-            raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp_eom})")
+        sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) after add_siw_dmft, or None for the context's own."""
+        if sigma_sum is None:
+            ptr = None
+        else:
+            ss = _zf(sigma_sum)
+            if ss.shape != (self.n, self.n, self.nf, self.nkp_eom):
+                raise ValueError(f"sigma_sum must be ({self.n}, {self.n}, {self.nf}, {self.nkp_eom})")
+            ptr = _dptr(ss)
         ndk = np.zeros((self.nkp_eom, self.n, self.n), dtype=np.complex128, order="F")
         nd = np.zeros(self.n, dtype=np.complex128)
-        self._check(lib.adga_get_ndga(self.h, _dptr(ss), _dptr(ndk), _dptr(nd)))
+        self._check(lib.adga_get_ndga(self.h, ptr, _dptr(ndk), _dptr(nd)))
         return ndk, nd
 
     # ----------------------------------------------------------------- derivatives (tangent-linear mode)

@@ -862,8 +862,22 @@ int adga_get_nfock(adga_ctx* c, double* n_fock) {
   });
 }
 
+// The context's own sigma_sum in `ds` (n2 * 2 iwfmax_small * nkp_eom): the accumulated Sigma_nl (the pending
+// EOM convolved, the negative frequencies filled as adga_get_sigma does) plus siw on the orbital diagonal,
+// as add_siw_dmft forms it (src/eom_module.f90:237-251, src/main.F90:869) - a device copy instead of the
+// caller's array crossing PCIe.
+void own_sigma_sum(adga_ctx* c, z64* ds) {
+  const Geom& g = c->g;
+  eom_flush(c);
+  adga::launch_fill_neg(g, g.nkp_eom, c->sig_nl, c->stream);
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(ds, c->sig_nl, sizeof(z64) * g.n2 * g.nf * g.nkp_eom, hipMemcpyDeviceToDevice, c->stream));
+  adga::launch_add_siw(g, g.nkp_eom, c->siw, ds, c->stream);
+  CK(hipGetLastError());
+}
+
 int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc, double* gloc) {
-  if (!c || !sigma_sum) return ADGA_E_ARG;
+  if (!c) return ADGA_E_ARG;
   const Geom& g = c->g;
   if (g.nkp_eom != g.nkp) {
     c->err = "get_sigma_g_loc: sigma_sum must cover the whole k grid (nkp_eom == nkp)";
@@ -872,7 +886,10 @@ int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc
   return guarded(c, [&]() -> int {
     const size_t ns = (size_t)g.n2 * g.nf * g.nkp, nl = (size_t)g.n2 * g.nf, ng = (size_t)2 * g.iwmax * g.n2;
     DevBuf<> ds(ns), dl(nl), dg(ng), dw(adga::op_sigma_loc_scratch(g));
-    upload(ds.p, sigma_sum, ns, c->stream);
+    if (sigma_sum)
+      upload(ds.p, sigma_sum, ns, c->stream);
+    else
+      own_sigma_sum(c, ds.p);
     const int rc = op_finish(
         c, adga::launch_op_sigma_g_loc(g, c->hk, c->siw, c->dc, ds.p, dl.p, dg.p, dw.p, c->stream), "get_sigma_g_loc");
     if (rc != ADGA_OK) return rc;
@@ -884,12 +901,15 @@ int adga_get_sigma_g_loc(adga_ctx* c, const double* sigma_sum, double* sigma_loc
 }
 
 int adga_get_ndga(adga_ctx* c, const double* sigma_sum, double* n_dga_k, double* n_dga) {
-  if (!c || !sigma_sum) return ADGA_E_ARG;
+  if (!c) return ADGA_E_ARG;
   return guarded(c, [&]() -> int {
     const Geom& g = c->g;
     const size_t ns = (size_t)g.n2 * g.nf * g.nkp_eom, nk = (size_t)g.nkp_eom * g.n2;
     DevBuf<> ds(ns), dk(nk), dn((size_t)g.n);
-    upload(ds.p, sigma_sum, ns, c->stream);
+    if (sigma_sum)
+      upload(ds.p, sigma_sum, ns, c->stream);
+    else
+      own_sigma_sum(c, ds.p);
     const int rc = op_finish(
         c, adga::launch_op_ndga(g, c->hk, c->siw, c->dc, c->keom, ds.p, dk.p, dn.p, c->stream), "get_ndga");
     if (rc != ADGA_OK) return rc;

@@ -110,12 +110,18 @@ void launch_smat(const Geom& g, int npts, const z64* chi0q, const z64* winv, z64
 // columns [N, ldm) are not written (never read).  The complex64 build (the refined path's LU
 // input, whose residuals are formed in FP64 from S and C themselves) computes in FP32: its entries
 // then differ from the rounded FP64 ones by a few FP32 ulps.  The complex128 build computes in FP64.
+// Frequency blocks per workgroup (DB) and points per workgroup (PB, sharing each C row read).  For
+// n <= 3 a workgroup's row segment is DB * n2 elements: complex64 takes 16 blocks, so that at n = 3 each
+// segment is 1152 bytes and starts on a 128-byte line (8 blocks gave 576-byte segments, every other one
+// starting mid-line; 21.2 -> 20.3 ms per build of 8000 matrices at N = 1080), with 2 points per
+// workgroup to keep four workgroups per CU; complex128 segments of 8 blocks are already whole lines.
 template <int NN, int ES>
 struct BuildShape {
   static constexpr int N2 = NN * NN, N4 = N2 * N2;
-  static constexpr int DB = NN <= 3 ? 8 : 2;  // frequency blocks per workgroup
+  static constexpr int DB = NN <= 3 ? (ES == 8 ? 16 : 8) : 2;
   static constexpr int PB0 = 40960 / (DB * N4 * ES);
-  static constexpr int PB = PB0 < 1 ? 1 : PB0 > 4 ? 4 : PB0;  // points per workgroup (share each C row read)
+  static constexpr int PB1 = PB0 < 1 ? 1 : PB0 > 4 ? 4 : PB0;
+  static constexpr int PB = (ES == 8 && NN <= 3 && PB1 > 2) ? 2 : PB1;
 };
 
 template <typename T, int NN>
@@ -126,9 +132,11 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
   using Sh = BuildShape<NN, (int)sizeof(CA)>;
   constexpr int N2 = Sh::N2, N4 = Sh::N4, DB = Sh::DB, PB = Sh::PB;
   __shared__ CA sS[PB][DB * N4];
-  const int ch = blockIdx.y, b0 = blockIdx.z * PB, t = threadIdx.x;
+  // grid (point groups, channels, frequency blocks): the point group runs fastest, so the workgroups
+  // resident at one time read the same column block of C, which stays in their XCDs' L2
+  const int ch = blockIdx.y, b0 = blockIdx.x * PB, t = threadIdx.x;
   const int N = g.N;
-  const int d0 = blockIdx.x * DB;
+  const int d0 = blockIdx.z * DB;
   const int nd = min(DB, g.nf - d0);
   const int np = min(PB, npts - b0);
   for (int pb = 0; pb < np; ++pb) {
@@ -217,7 +225,7 @@ void launch_build(const Geom& g, int npts, int ldm, const z64* S, const z64* cd,
   case NN: {                                                                                           \
     using Sh = BuildShape<NN, (int)sizeof(cplx<T>)>;                                                   \
     constexpr int DB = Sh::DB, PB = Sh::PB;                                                           \
-    dim3 grid((g.nf + DB - 1) / DB, 2, (npts + PB - 1) / PB);                                         \
+    dim3 grid((npts + PB - 1) / PB, 2, (g.nf + DB - 1) / DB);                                         \
     hipLaunchKernelGGL((k_build<T, NN>), grid, dim3(256), 0, s, g, npts, ldm, S, cd, cm, M);          \
     break;                                                                                             \
   }

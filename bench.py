@@ -157,22 +157,27 @@ def one_particle(api, synth, iwmax=2000, reps=3, cpu_iwmax=200, cpu=True):
     ss = np.asfortranarray(ss)
     calls = {"get_giw": lambda c: c.get_giw(), "get_nfock": lambda c: c.get_nfock(),
              "get_sigma_g_loc": lambda c: c.get_sigma_g_loc(ss), "get_ndga": lambda c: c.get_ndga(ss)}
-    ms = {}
+    # the same two routines with sigma_sum = NULL: the context's own self-energy, formed on the device
+    own = {"get_sigma_g_loc": lambda c: c.get_sigma_g_loc(), "get_ndga": lambda c: c.get_ndga()}
+    ms, ms_own = {}, {}
     with api.Context(inp) as ctx:
-        for name, f in calls.items():
-            f(ctx)
-            best = 1e30
-            for _ in range(reps):
-                t0 = time.perf_counter()
+        for res, fns in ((ms, calls), (ms_own, own)):
+            for name, f in fns.items():
                 f(ctx)
-                best = min(best, time.perf_counter() - t0)
-            ms[name] = round(best * 1e3, 3)
+                best = 1e30
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    f(ctx)
+                    best = min(best, time.perf_counter() - t0)
+                res[name] = round(best * 1e3, 3)
     inv = 4 * nkp * iwmax
     out = {"workload": f"t2g_k20 grid (8000 k), iwmax = {iwmax} (k x nu inversions per routine: {nkp * iwmax})",
            "ms": ms, "total_ms": round(sum(ms.values()), 3),
            "inversions_per_s": round(inv / (sum(ms.values()) / 1e3), 1),
+           "ms_context_sigma": ms_own,
            "note": "wall time per call through the C ABI; get_sigma_g_loc and get_ndga include the upload of "
-                   "sigma_sum (n^2 x 2 iwfmax_small x nkp complex, 138 MB) from host memory"}
+                   "sigma_sum (n^2 x 2 iwfmax_small x nkp complex, 138 MB) from host memory; ms_context_sigma: "
+                   "the same two with sigma_sum = NULL, the context's own Sigma_nl + siw formed on the device"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ref_runner
     if cpu and ref_runner.op_available():

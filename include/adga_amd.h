@@ -298,7 +298,9 @@ int adga_get_nfock(adga_ctx *ctx, double *n_fock);
 
 /*
  * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp): the self-energy on the whole k grid after
- * add_siw_dmft (requires nkp_eom == nkp, as the reference's call does).  Outputs (may be NULL):
+ * add_siw_dmft (requires nkp_eom == nkp, as the reference's call does).  NULL: the context's own,
+ * formed on the device from the Sigma_nl it has accumulated (as adga_get_sigma returns it) plus siw
+ * on the orbital diagonal, without crossing PCIe (a single-process run's sigma_sum).  Outputs (may be NULL):
  * sigma_loc(ndim, ndim, 2*iwfmax_small) = its k mean; gloc(2*iwmax, ndim, ndim) = (1/nkp) sum_k G
  * with Sigma = sigma_sum inside the box (iw < iwfmax_small) and siw outside, gloc(-nu-1) = gloc(nu)^H.
  * Replaces: get_sigma_g_loc (src/one_particle_quant_module.f90:124-180).
@@ -306,7 +308,8 @@ int adga_get_nfock(adga_ctx *ctx, double *n_fock);
 int adga_get_sigma_g_loc(adga_ctx *ctx, const double *sigma_sum, double *sigma_loc, double *gloc);
 
 /*
- * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) as above, on the EOM k list.  Outputs (may be NULL):
+ * sigma_sum(ndim, ndim, 2*iwfmax_small, nkp_eom) as above, on the EOM k list (NULL: the context's
+ * own, as above).  Outputs (may be NULL):
  * n_dga_k(nkp_eom, ndim, ndim) = 2/beta sum_iw Re G(k_data_eom(ik), nu) + 1/2 on the diagonal
  * (Sigma = sigma_sum inside the box, siw outside); n_dga(ndim) = mean over ik of its diagonal.
  * Replaces: get_ndga (src/one_particle_quant_module.f90:363-393).

@@ -84,3 +84,29 @@ def test_gpu_get_giw_replaces_the_context_giw():
         with pytest.raises(api.AdgaError) as e:
             ctx.run_points(np.array([1], dtype=np.int32))
         assert e.value.code == 6  # ADGA_E_STATE
+
+
+def test_gpu_sigma_sum_of_the_context():
+    """adga_get_sigma_g_loc / adga_get_ndga with sigma_sum = NULL take the context's own: the Sigma_nl it
+    accumulated over the t2g_k4 loop plus siw on the orbital diagonal, formed on the device as
+    add_siw_dmft forms it (src/eom_module.f90:237-251).  Equal, exactly, to the same routines given that
+    array from the host, also on a second call."""
+    from adga_amd import api
+    from golden.make_golden import FIXTURES
+    gold = load_golden("t2g_k4")
+    wl = synth.workload(FIXTURES["t2g_k4"][0])
+    inp = synth.build_inputs(wl)
+    pts = [tuple(p) for p in gold["points"].tolist()]
+    n, iwf, iwmax = wl.n, wl.iwfmax_small, inp.siw.shape[0] // 2
+    print("sigma_sum of the context (t2g_k4 loop)", flush=True)
+    with api.Context(inp) as ctx:
+        out = api.run_loop(ctx, pts, lambda w, c: synth.chi_loc(inp, w, c), "fp64")
+        ss = np.array(out["sigma_nl"], order="F")
+        for i in range(n):
+            ss[i, i, :, :] += inp.siw[iwmax - iwf:iwmax + iwf, i][:, None]
+        assert np.abs(out["sigma_nl"]).max() > 0
+        host = ctx.get_sigma_g_loc(ss) + ctx.get_ndga(ss)
+        for _ in range(2):
+            own = ctx.get_sigma_g_loc() + ctx.get_ndga()
+            for a, b in zip(host, own):
+                assert np.array_equal(a, b)
