@@ -149,7 +149,10 @@ void launch_bubble_gather(const Geom& g, int npts, const int* qidx, const z64* a
 // the bubble's derivative for every q of an omega from the transforms of G and dG (product rule)
 void launch_bubble_all_tangent(const Geom& g, int iwb, const z64* tw, const z64* ghat, const z64* dghat, z64* P,
                                z64* out, hipStream_t s);
-void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s);
+// distinct: the batch's q indices are pairwise different (then one addition per element runs in parallel
+// over the points; else the points are added in order by one thread per element)
+void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s,
+                        bool distinct = false);
 // derivatives carried by the EOM flush: per direction d, the accumulator of dm_d on the q grid (macc + d*nm),
 // the k-grid transform of dG_d (dghat + d*ng) and d Sigma_nl (sigma + d*nsig); work: nm scratch
 struct EomTangent {

@@ -591,6 +591,14 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       qh.resize(nb);
       for (int b = 0; b < nb; ++b) qh[b] = c->q0[iq[p0 + b] - 1];
       CK(hipMemcpyAsync(c->qidx, qh.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+      // whether every q of the batch is different (the EOM scatter then runs over the points in parallel)
+      c->qmark.resize(c->g.nkp);
+      c->q_distinct = true;
+      for (int b = 0; b < nb; ++b) {
+        if (c->qmark[qh[b]]) c->q_distinct = false;
+        c->qmark[qh[b]] = 1;
+      }
+      for (int b = 0; b < nb; ++b) c->qmark[qh[b]] = 0;
       if (c->vq) {
         ensure_vq_batch(c, nb);
         qp.resize(nb);
@@ -742,7 +750,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
             CK(hipMemsetAsync(c->macc, 0, sizeof(z64) * nm, s));
           }
           if (c->eom_pending && (c->eom_iwb != c->iwb || c->eom_g.iwf != g.iwf)) eom_flush(c);
-          adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s);
+          adga::launch_eom_scatter(g, nb, c->qidx, c->mt, c->macc, s, c->q_distinct);
           c->eom_pending = true;
           c->eom_iwb = c->iwb;
           c->eom_g = g;

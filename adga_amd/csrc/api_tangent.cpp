@@ -118,7 +118,8 @@ void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>
     // d m_tot is accumulated on the q grid beside m_tot and both terms are formed when the omega's EOM is
     // flushed; else as direct sums per batch
     if (c->eom_mode == 1 && c->t_macc) {
-      adga::launch_eom_scatter(g, nb, c->qidx, c->t_dmt, c->t_macc + dir * (size_t)G.iwf * G.n4 * G.nkp, s);
+      adga::launch_eom_scatter(g, nb, c->qidx, c->t_dmt, c->t_macc + dir * (size_t)G.iwf * G.n4 * G.nkp, s,
+                               c->q_distinct);
     } else {
       z64* sig = c->t_sig + dir * nsig;
       adga::launch_eom_direct(g, c->iwb, nb, c->qidx, c->keom, c->t_dmt, c->gt, sig, s);

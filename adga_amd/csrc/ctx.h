@@ -83,6 +83,8 @@ struct adga_ctx {
   void* inv = nullptr;  // per-matrix panel inverses of the LU
   double* rel = nullptr;
   int *piv = nullptr, *info = nullptr, *qidx = nullptr;
+  bool q_distinct = false;          // the current batch's q indices are pairwise different
+  std::vector<unsigned char> qmark;  // host scratch of that test, zero between batches (nkp entries)
   int *act = nullptr, *keep = nullptr;  // per-matrix refinement flags, per-point "enters the self-energy" flags
   FallbackBufs fb;
   // LU operation lists (factor + solve of the appended right-hand sides; solve only), per precision

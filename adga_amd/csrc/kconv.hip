@@ -168,7 +168,42 @@ __global__ void k_eom_scatter(Geom g, int npts, const int* __restrict__ qidx, co
   }
 }
 
-void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s) {
+// The same when the batch's points are distinct (each q at most once): a workgroup takes pt points at one
+// nu, reads their n^4 values of m_tot at that nu (contiguous per point, index i1 + n2 (l n + k')) into LDS
+// with coalesced loads, and adds them to Macc with consecutive points on consecutive lanes, so that the
+// consecutive q of a usual batch give whole-line accesses; every element of Macc gets the same single
+// addition as in k_eom_scatter.
+__global__ void __launch_bounds__(256) k_eom_scatter_t(Geom g, int npts, int pt, const int* __restrict__ qidx,
+                                                       const z64* __restrict__ mt, z64* __restrict__ macc) {
+  extern __shared__ z64 tile[];  // [pt][n4 + 1]
+  const int nu = blockIdx.y, b0 = blockIdx.x * pt, t = threadIdx.x;
+  const int n2 = g.n2, n4 = g.n4;
+  const int np = min(pt, npts - b0);
+  const long per = (long)n2 * g.N;
+  const long off = (long)n4 * (g.iwf + nu);  // the (point, nu) block of m_tot
+  for (int e = t; e < np * n4; e += blockDim.x) {
+    const int pb = e / n4, f = e % n4;
+    tile[pb * (n4 + 1) + f] = mt[(long)(b0 + pb) * per + off + f];
+  }
+  __syncthreads();
+  for (int e = t; e < n4 * pt; e += blockDim.x) {
+    const int pb = e % pt, fm = e / pt;  // fm = i1 n2 + l n + k', the field within nu
+    if (pb >= np) continue;
+    const int i1 = fm / n2, lk = fm % n2;
+    z64* dst = macc + ((long)nu * n4 + fm) * g.nkp + qidx[b0 + pb];
+    *dst = cadd(*dst, tile[pb * (n4 + 1) + lk * n2 + i1]);
+  }
+}
+
+void launch_eom_scatter(const Geom& g, int npts, const int* qidx, const z64* mt, z64* macc, hipStream_t s,
+                        bool distinct) {
+  if (distinct) {
+    const int pt = std::max(1, std::min(32, 2688 / (g.n4 + 1)));  // <= 42 KB of LDS
+    const size_t lds = (size_t)pt * (g.n4 + 1) * sizeof(z64);
+    hipLaunchKernelGGL(k_eom_scatter_t, dim3((unsigned)((npts + pt - 1) / pt), (unsigned)g.iwf), dim3(256), lds, s, g,
+                       npts, pt, qidx, mt, macc);
+    return;
+  }
   long nfield = (long)g.iwf * g.n4;
   hipLaunchKernelGGL(k_eom_scatter, dim3((unsigned)((nfield + 63) / 64)), dim3(64), 0, s, g, npts, qidx, mt, macc);
 }

@@ -371,3 +371,22 @@ def test_gpu_near_instability_falls_back_to_fp64():
         e_lim = rel_err(res["limited"][k], res["fp64"][k])
         print(f"  {k}: fp64 vs oracle {e_or:.1e}, refined vs fp64 {e_rf:.1e}, limited vs fp64 {e_lim:.1e}", flush=True)
         assert e_or < 1e-6 and e_rf < 1e-6 and e_lim < 1e-6, k
+
+
+def test_gpu_repeated_q_in_a_batch():
+    """A batch whose q list repeats points (q = 5, 9, 5, 17, 9 at omega = 0 and 2): every repeat counts, as
+    in the reference's loop over the list.  The k-grid EOM (its scatter then adds the points in order) and
+    the direct EOM sums give the same Sigma_nl, and chi of a repeated point equals its first evaluation."""
+    api = _api()
+    wl = synth.workload("t2g_k4")
+    inp = synth.build_inputs(wl)
+    pts = [(w, q) for w in (0, 2) for q in (5, 9, 5, 17, 9)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    res = {}
+    for eom in (1, 0):
+        with api.Context(inp, batch=64, eom=eom) as ctx:
+            res[eom] = api.run_loop(ctx, pts, cl, "fp64")
+    assert rel_err(res[1]["sigma_nl"], res[0]["sigma_nl"]) < 1e-12
+    for k in ("chi_d", "chi_m", "bubble_nl"):
+        assert np.array_equal(res[1][k][:, :, 2], res[1][k][:, :, 0]), k
+        assert np.array_equal(res[1][k][:, :, 4], res[1][k][:, :, 1]), k
