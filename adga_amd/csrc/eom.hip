@@ -49,6 +49,58 @@ __global__ void k_chi(Geom g, int npts, const z64* __restrict__ chi0q, const z64
   if (ch == 0) bub_out[(long)b * g.n4 + e] = csub(cscale(bs, ib2), bubloc[e]);
 }
 
+// The same for n <= 3 with one workgroup per point: thread (k slice, channel, row s) accumulates row s of
+// chi_ch over its slice of x = d*n2 + i3 (the n2 values of chi0^q at one x are read by every thread of the
+// slice), the slices are summed through LDS, and threads 0 .. n4-1 form bubble_nl (summed over d in the
+// same order as k_chi).  The chi sums run in another order than k_chi's: roundoff-level differences.
+template <int N2>
+__global__ void __launch_bounds__(256) k_chi_wg(Geom g, const z64* __restrict__ chi0q, const z64* __restrict__ chi0w,
+                                                const z64* __restrict__ gam_d, const z64* __restrict__ gam_m,
+                                                const z64* __restrict__ eta, const z64* __restrict__ bubloc,
+                                                z64* __restrict__ chi_out, z64* __restrict__ bub_out) {
+  constexpr int N4 = N2 * N2, KS = 256 / (2 * N2);
+  __shared__ z64 part[KS][2][N4];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int ks = t / (2 * N2), ch = (t % (2 * N2)) / N2, s = t % N2;
+  const int N = g.N;
+  const z64* cq = chi0q + (long)b * g.nf * N4;
+  if (ks < KS) {
+    const z64* gam = ch == 0 ? gam_d : gam_m;
+    const z64* et = eta + (long)(2 * b + ch) * N2 * N;
+    z64 acc[N2];
+#pragma unroll
+    for (int s2 = 0; s2 < N2; ++s2) acc[s2] = mk(0.0, 0.0);
+    const int x0 = (int)((long)N * ks / KS), x1 = (int)((long)N * (ks + 1) / KS);
+    for (int x = x0; x < x1; ++x) {
+      const int d = x / N2, i3 = x % N2;
+      const z64 gv = gam[s + (long)x * N2], ev = et[s + (long)x * N2];
+      const z64 w = chi0w[i3 + d * N2];  // chi0^w enters chi0^q - chi0^w where s2 == i3
+      const z64* cx = cq + (long)d * N4 + i3;
+#pragma unroll
+      for (int s2 = 0; s2 < N2; ++s2) {
+        const z64 c = cx[s2 * N2];
+        acc[s2] = cfma(gv, s2 == i3 ? csub(c, w) : c, acc[s2]);
+        acc[s2] = cfma(ev, c, acc[s2]);
+      }
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < N2; ++s2) part[ks][ch][s + s2 * N2] = acc[s2];
+  }
+  __syncthreads();
+  const double ib2 = 1.0 / (g.beta * g.beta);
+  for (int e = t; e < 2 * N4; e += 256) {
+    const int c2 = e / N4, f = e % N4;
+    z64 v = mk(0.0, 0.0);
+    for (int k = 0; k < KS; ++k) v = cadd(v, part[k][c2][f]);
+    chi_out[((long)b * 2 + c2) * N4 + f] = cscale(v, ib2);
+  }
+  for (int e = t; e < N4; e += 256) {
+    z64 bs = mk(0.0, 0.0);
+    for (int d = 0; d < g.nf; ++d) bs = cadd(bs, cq[(long)d * N4 + e]);
+    bub_out[(long)b * N4 + e] = csub(cscale(bs, ib2), bubloc[e]);
+  }
+}
+
 // m_tot = -(U + V - U~/2) eta_d /(beta nqp) + 1.5 U~ eta_m /(beta nqp) + U gamma^q_d /(beta nqp)
 //        - V gamma^w_d /(beta nqp)   (eom_module.f90:183-207; the V terms only with V(q): vq non-null).
 // Output (n2, N) Fortran per point.
@@ -174,6 +226,16 @@ void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, cons
                     z64* chi_out, z64* bub_out, hipStream_t s) {
   long tot = (long)2 * npts * g.n2 * g.N;
   hipLaunchKernelGGL(k_eta, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, 2 * npts, Y, opgd, opgm, eta);
+  if (g.n <= 3) {
+#define ADGA_CHI_WG(NN)                                                                                          \
+  if (g.n == NN) {                                                                                              \
+    hipLaunchKernelGGL(k_chi_wg<NN * NN>, dim3((unsigned)npts), dim3(256), 0, s, g, chi0q, chi0w, gamd, gamm, eta, \
+                       bubloc, chi_out, bub_out);                                                               \
+    return;                                                                                                     \
+  }
+    ADGA_CHI_WG(1) ADGA_CHI_WG(2) ADGA_CHI_WG(3)
+#undef ADGA_CHI_WG
+  }
   long t2 = (long)npts * 2 * g.n4;
   hipLaunchKernelGGL(k_chi, dim3((unsigned)((t2 + 127) / 128)), dim3(128), 0, s, g, npts, chi0q, chi0w, gamd, gamm,
                      eta, bubloc, chi_out, bub_out);
