@@ -71,6 +71,7 @@ __global__ void __launch_bounds__(256) k_chi_wg(Geom g, const z64* __restrict__ 
 #pragma unroll
     for (int s2 = 0; s2 < N2; ++s2) acc[s2] = mk(0.0, 0.0);
     const int x0 = (int)((long)N * ks / KS), x1 = (int)((long)N * (ks + 1) / KS);
+#pragma unroll 2
     for (int x = x0; x < x1; ++x) {
       const int d = x / N2, i3 = x % N2;
       const z64 gv = gam[s + (long)x * N2], ev = et[s + (long)x * N2];
@@ -96,6 +97,7 @@ __global__ void __launch_bounds__(256) k_chi_wg(Geom g, const z64* __restrict__ 
   }
   for (int e = t; e < N4; e += 256) {
     z64 bs = mk(0.0, 0.0);
+#pragma unroll 8
     for (int d = 0; d < g.nf; ++d) bs = cadd(bs, cq[(long)d * N4 + e]);
     bub_out[(long)b * N4 + e] = csub(cscale(bs, ib2), bubloc[e]);
   }
