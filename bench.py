@@ -252,6 +252,52 @@ def shrinking_box(api, synth, batch=128, omegas=(0, 5, 20), warmup=3, steps=1):
     return res
 
 
+def small_workload(api, synth, cpu=True, procs=16):
+    """The 1-orbital Hubbard workload the reference runs on the CPU (sc1_k8: simple cubic, t = 0.25, U = 2,
+    beta = 10, half filling, 8^3 k with q on the same grid, 2x20 fermionic x 11 bosonic frequencies, N = 40):
+    all 5632 points, omega by omega in batches of the whole q grid, through the GPU path in both precisions
+    (one untimed pass first, then a timed pass from reset_sigma to get_sigma), and the reference's own
+    modules (oracle/_ref) over the same points on `procs` host cores, one single-threaded process per
+    contiguous block of points (the reference's rank-per-core split), whole-job wall time including each
+    process's set-up.  Launch- and latency-bound at this size: no roofline."""
+    wl = synth.workload("sc1_k8")
+    inp = synth.build_inputs(wl)
+    W, nq = wl.iwbmax_small, wl.nqp
+    omegas = list(range(-W, W + 1))
+    chi = {w: (synth.chi_loc(inp, w, "dens"), synth.chi_loc(inp, w, "magn")) for w in omegas}
+    iq = np.arange(1, nq + 1, dtype=np.int32)
+    res = {"workload": "sc1_k8: 1-orbital simple cubic, 8^3 k, 2x20 x 11 frequencies, N = 40, all points",
+           "points": nq * len(omegas)}
+    for precision in ("refined", "fp64"):
+        with api.Context(inp, batch=nq) as ctx:
+            for timed in (False, True):
+                ctx.reset_sigma()
+                t0 = time.perf_counter()
+                for w in omegas:
+                    ctx.set_local_omega(w, *chi[w])
+                    ctx.run_points(iq, precision)
+                ctx.get_sigma()
+                dt = time.perf_counter() - t0
+            res[precision] = {"seconds": round(dt, 4), "points_per_s": round(res["points"] / dt, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ref_runner
+        from concurrent.futures import ThreadPoolExecutor
+        if ref_runner.available():
+            pts = [(w, q) for w in omegas for q in range(nq)]
+            blocks = [pts[len(pts) * i // procs:len(pts) * (i + 1) // procs] for i in range(procs)]
+            cl = lambda w, c: chi[w][0 if c == "dens" else 1]  # noqa: E731
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(procs) as ex:
+                walls = list(ex.map(lambda b: ref_runner.run(inp, b, cl, threads=1)["wall_s"], blocks))
+            job = time.perf_counter() - t0
+            res["cpu_reference"] = {"seconds": round(job, 3), "points_per_s": round(res["points"] / job, 1),
+                                    "cores": procs, "slowest_process_s": round(max(walls), 3),
+                                    "kind": "reference"}
+            res["gpu_refined_over_cpu"] = round(res["refined"]["points_per_s"] / res["cpu_reference"]["points_per_s"], 1)
+    return res
+
+
 def gradient(synth):
     """The differentiation workload (3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all 36288
     points): the forward pass (complex128) and the backward pass (forward + tangent-linear along t, t', U)
@@ -311,6 +357,10 @@ def main():
                          "and print their JSON object")
     ap.add_argument("--no-shrinking-box", action="store_true", help="skip the shrinking-box (5-orbital) timing")
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient (forward + tangent) timing")
+    ap.add_argument("--no-small-workload", action="store_true",
+                    help="skip the 1-orbital 8^3 workload (all 5632 points, GPU and CPU reference)")
+    ap.add_argument("--only-small-workload", action="store_true",
+                    help="time only the 1-orbital 8^3 workload and print its JSON object")
     ap.add_argument("--only-shrinking-box", action="store_true",
                     help="time only the shrinking-box workload and print its JSON object")
     ap.add_argument("--only-gradient", action="store_true", help="time only the gradient pass and print its JSON object")
@@ -335,6 +385,10 @@ def main():
         return
     if args.only_gradient:
         print(json.dumps({"gradient": gradient(synth)}), flush=True)
+        return
+    if args.only_small_workload:
+        print(json.dumps({"small_workload": small_workload(api, synth, cpu=not args.no_cpu_baseline,
+                                                           procs=args.cpu_procs)}), flush=True)
         return
     wl = synth.workload(args.workload)
     inp = synth.build_inputs(wl)
@@ -421,6 +475,8 @@ def main():
         line["shrinking_box"] = shrinking_box(api, synth)
     if not args.no_gradient:
         line["gradient"] = gradient(synth)
+    if not args.no_small_workload:
+        line["small_workload"] = small_workload(api, synth, cpu=not args.no_cpu_baseline, procs=args.cpu_procs)
     print(json.dumps(line), flush=True)
     if line.get("parity_ok") is False:
         print("refined results differ from the plain FP64 path by more than 1e-10 (or points failed)",
