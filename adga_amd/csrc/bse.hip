@@ -350,7 +350,79 @@ __global__ void k_gamq_z(Geom g, int npts, const z64* __restrict__ S, z64* __res
   Z[idx] = S[((long)b * g.nf + d) * g.n4 + s * g.n2 + ip];
 }
 
+// k_resid_z and k_gamq_z for n <= 3 with one thread per (channel, d, point b, column s) forming the n2 rows
+// x = d*n2 + i' of its column: each thread reads its values of V (or of S_b) once, and for each i' the
+// wave writes a contiguous run of Z's row x (consecutive lanes hold consecutive columns b*n2 + s).  The
+// same sums in the same order as the one-output-per-thread forms above.
+template <int N2>
+__global__ void __launch_bounds__(256) k_resid_zt(Geom g, int npts, const z64* __restrict__ S,
+                                                  const z64* __restrict__ Y, const z64* __restrict__ opg_d,
+                                                  const z64* __restrict__ opg_m, z64* __restrict__ Z) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ncol = (long)npts * N2;
+  if (idx >= 2L * g.nf * ncol) return;
+  const int col = (int)(idx % ncol);
+  const int r = (int)(idx / ncol), d = r % g.nf, ch = r / g.nf;
+  const int b = col / N2, s = col - b * N2;
+  const z64* Sd = S + ((long)b * g.nf + d) * (N2 * N2);
+  z64 v[N2];
+  if (Y) {
+    const z64* Ym = Y + (long)(2 * b + ch) * g.N * N2;
+#pragma unroll
+    for (int cp = 0; cp < N2; ++cp) v[cp] = Ym[(long)(d * N2 + cp) * N2 + s];
+  } else {
+    const z64* opg = ch == 0 ? opg_d : opg_m;
+#pragma unroll
+    for (int cp = 0; cp < N2; ++cp) v[cp] = opg[s + (long)(d * N2 + cp) * N2];
+  }
+  z64* Zc = Z + ((long)ch * g.N + (long)d * N2) * ncol + col;
+#pragma unroll
+  for (int ip = 0; ip < N2; ++ip) {
+    z64 acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int cp = 0; cp < N2; ++cp) acc = cfma(Sd[cp * N2 + ip], v[cp], acc);
+    Zc[(long)ip * ncol] = acc;
+  }
+}
+
+template <int N2>
+__global__ void __launch_bounds__(256) k_gamq_zt(Geom g, int npts, const z64* __restrict__ S, z64* __restrict__ Z) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ncol = (long)npts * N2;
+  if (idx >= (long)g.nf * ncol) return;
+  const int col = (int)(idx % ncol), d = (int)(idx / ncol);
+  const int b = col / N2, s = col - b * N2;
+  const z64* Sd = S + ((long)b * g.nf + d) * (N2 * N2) + s * N2;
+  z64* Zc = Z + (long)d * N2 * ncol + col;
+#pragma unroll
+  for (int ip = 0; ip < N2; ++ip) Zc[(long)ip * ncol] = Sd[ip];
+}
+
+// k_resid_z / k_resid_zt over both channels: Y (a solution) or, with Y null, (1 + gamma^w)^T
+static void resid_z(const Geom& g, int npts, const z64* S, const z64* Y, const z64* opgd, const z64* opgm, z64* Z,
+                    hipStream_t s) {
+  const long tz = 2L * g.nf * npts * g.n2;
+#define ADGA_RZ(NN)                                                                                                 \
+  if (g.n == NN) {                                                                                                 \
+    hipLaunchKernelGGL(k_resid_zt<NN * NN>, dim3((unsigned)((tz + 255) / 256)), dim3(256), 0, s, g, npts, S, Y,     \
+                       opgd, opgm, Z);                                                                             \
+    return;                                                                                                        \
+  }
+  ADGA_RZ(1) ADGA_RZ(2) ADGA_RZ(3)
+#undef ADGA_RZ
+  const long tot = 2L * npts * g.N * g.n2;
+  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Y, opgd, opgm, Z);
+}
+
 void launch_gamq_z(const Geom& g, int npts, const z64* S, z64* Z, hipStream_t s) {
+  const long tz = (long)g.nf * npts * g.n2;
+#define ADGA_GZ(NN)                                                                                                 \
+  if (g.n == NN) {                                                                                                 \
+    hipLaunchKernelGGL(k_gamq_zt<NN * NN>, dim3((unsigned)((tz + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);     \
+    return;                                                                                                        \
+  }
+  ADGA_GZ(1) ADGA_GZ(2) ADGA_GZ(3)
+#undef ADGA_GZ
   long tot = (long)npts * g.N * g.n2;
   hipLaunchKernelGGL(k_gamq_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Z);
 }
@@ -366,10 +438,7 @@ void launch_gamq(const Geom& g, int npts, const z64* S, const z64* cd, z64* Z, z
 // (1+gamma^w_d)^T (W^T (1+gamma^w_d)^T).
 void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* opgd,
                 const z64* opgm, z64* Z, z64* B, hipStream_t s, const VqTerm* vq) {
-  int nmat = 2 * npts;
-  long tot = (long)nmat * g.N * g.n2;
-  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, nullptr, opgd,
-                     opgm, Z);
+  resid_z(g, npts, S, nullptr, opgd, opgm, Z, s);
   resid_gemm(g, npts, 0, 2, cd, cm, Z, B, nullptr, nullptr, s);
   if (vq) launch_vq_add(g, npts, *vq, nullptr, B, s);
 }
@@ -378,10 +447,8 @@ void launch_rhs(const Geom& g, int npts, const z64* S, const z64* cd, const z64*
 // minus (1 + gamma^w_d)^T (W^T Y) in the density channel).
 void launch_residual(const Geom& g, int npts, const z64* S, const z64* cd, const z64* cm, const z64* B,
                      const z64* Y, z64* Z, z64* R, double* rel, hipStream_t s, const VqTerm* vq) {
-  int nmat = 2 * npts;
-  long tot = (long)nmat * g.N * g.n2;
-  hipLaunchKernelGGL(k_resid_z, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, S, Y, nullptr,
-                     nullptr, Z);
+  const int nmat = 2 * npts;
+  resid_z(g, npts, S, Y, nullptr, nullptr, Z, s);
   resid_gemm(g, npts, 0, 2, cd, cm, Z, R, B, Y, s);
   if (vq) launch_vq_add(g, npts, *vq, Y, R, s);
   hipLaunchKernelGGL(k_resid_norm, dim3(nmat), dim3(256), 0, s, g, R, B, rel);

@@ -243,8 +243,72 @@ void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, cons
                      eta, bubloc, chi_out, bub_out);
 }
 
+// k_mtot for n <= 3 with one thread per (point b, x) forming the n2 values s of its row: eta_d, eta_m and
+// gamma^q_d at x (and gamma^w_d with V(q)) are read once per thread instead of once per output; the same
+// sums in the same order as k_mtot.
+template <int N2, bool VQ>
+__global__ void __launch_bounds__(256) k_mtot_t(Geom g, int npts, const z64* __restrict__ u,
+                                                const z64* __restrict__ ut, const z64* __restrict__ eta,
+                                                const z64* __restrict__ gamq, z64* __restrict__ mt,
+                                                const int* __restrict__ qpos, const z64* __restrict__ vq,
+                                                const z64* __restrict__ gamd, int accumulate) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)npts * g.N) return;
+  const int b = (int)(idx / g.N), x = (int)(idx - (long)b * g.N);
+  const long per = (long)N2 * g.N;
+  const z64* ed = eta + (long)(2 * b) * per + (long)x * N2;
+  const z64* em = eta + (long)(2 * b + 1) * per + (long)x * N2;
+  const z64* gq = gamq + (long)(2 * b) * per + (long)x * N2;  // residual layout: point b at slot 2b
+  z64 e_d[N2], e_m[N2], g_q[N2], g_d[VQ ? N2 : 1];
+#pragma unroll
+  for (int a = 0; a < N2; ++a) {
+    e_d[a] = ed[a];
+    e_m[a] = em[a];
+    g_q[a] = gq[a];
+    if constexpr (VQ) g_d[a] = gamd[a + (long)x * N2];
+  }
+  const z64* v = VQ ? vq + (long)qpos[b] * (N2 * N2) : nullptr;
+  const double a1 = -1.0 / g.beta / g.nqp, a2 = 1.5 / g.beta / g.nqp, a3 = 1.0 / g.beta / g.nqp;
+  z64* out = mt + (long)b * per + (long)x * N2;
+#pragma unroll
+  for (int s = 0; s < N2; ++s) {
+    z64 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0), t4 = mk(0.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < N2; ++a) {
+      const z64 uu = u[s + a * N2], uw = ut[s + a * N2];
+      z64 uwork = mk(uu.re - 0.5 * uw.re, uu.im - 0.5 * uw.im);
+      if constexpr (VQ) {
+        const z64 va = v[s + a * N2];
+        uwork = cadd(va, uwork);
+        t4 = cfma(va, g_d[a], t4);
+      }
+      t1 = cfma(uwork, e_d[a], t1);
+      t2 = cfma(uw, e_m[a], t2);
+      t3 = cfma(uu, g_q[a], t3);
+    }
+    z64 m = cscale(t1, a1);
+    m = cadd(cscale(t2, a2), m);
+    m = cadd(cscale(t3, a3), m);
+    if constexpr (VQ) m = cadd(cscale(t4, a1), m);
+    out[s] = accumulate ? cadd(out[s], m) : m;
+  }
+}
+
 void launch_mtot(const Geom& g, int npts, const z64* u, const z64* ut, const z64* eta, const z64* gamq, z64* mt,
                  hipStream_t s, const int* qpos, const z64* vq, const z64* gamd, bool accumulate) {
+  const long tx = (long)npts * g.N;
+#define ADGA_MT(NN)                                                                                                 \
+  if (g.n == NN) {                                                                                                 \
+    if (vq)                                                                                                        \
+      hipLaunchKernelGGL((k_mtot_t<NN * NN, true>), dim3((unsigned)((tx + 255) / 256)), dim3(256), 0, s, g, npts,  \
+                         u, ut, eta, gamq, mt, qpos, vq, gamd, accumulate ? 1 : 0);                                 \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_mtot_t<NN * NN, false>), dim3((unsigned)((tx + 255) / 256)), dim3(256), 0, s, g, npts, \
+                         u, ut, eta, gamq, mt, qpos, vq, gamd, accumulate ? 1 : 0);                                 \
+    return;                                                                                                        \
+  }
+  ADGA_MT(1) ADGA_MT(2) ADGA_MT(3)
+#undef ADGA_MT
   long tot = (long)npts * g.n2 * g.N;
   hipLaunchKernelGGL(k_mtot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, npts, u, ut, eta, gamq, mt,
                      qpos, vq, gamd, accumulate ? 1 : 0);
