@@ -47,13 +47,81 @@ __global__ void __launch_bounds__(256) k_dft_axis(long nfield, int nx, int ny, i
   out[idx] = acc;
 }
 
+// The same with one thread per line (the NK outputs along `axis` of one field and one pair of the other
+// coordinates): the line's inputs are read once into registers instead of once per output, and consecutive
+// lanes take consecutive lines along the innermost of the other coordinates, so the passes along x and y
+// load and store whole rows.  Each output is the same sum, in the same order, as in k_dft_axis.
+template <int NK>
+__global__ void __launch_bounds__(256) k_dft_line(long nfield, int nx, int ny, int nz, int axis, int sign,
+                                                  const z64* __restrict__ tw, const z64* __restrict__ in,
+                                                  z64* __restrict__ out) {
+  const long per = (long)nx * ny * nz;
+  long lid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lid >= nfield * (per / NK)) return;
+  long base;
+  int stride;
+  const z64* w;
+  if (axis == 2) {  // lines along z (unit stride), lanes over y then x
+    const int y = (int)(lid % ny);
+    lid /= ny;
+    const int x = (int)(lid % nx);
+    base = (lid / nx) * per + (long)x * ny * nz + (long)y * nz;
+    stride = 1;
+    w = tw + nx * nx + ny * ny;
+  } else if (axis == 1) {  // along y, lanes over z then x
+    const int z = (int)(lid % nz);
+    lid /= nz;
+    const int x = (int)(lid % nx);
+    base = (lid / nx) * per + (long)x * ny * nz + z;
+    stride = nz;
+    w = tw + nx * nx;
+  } else {  // along x, lanes over z then y
+    const int z = (int)(lid % nz);
+    lid /= nz;
+    const int y = (int)(lid % ny);
+    base = (lid / ny) * per + (long)y * nz + z;
+    stride = ny * nz;
+    w = tw;
+  }
+  z64 v[NK];
+#pragma unroll
+  for (int k = 0; k < NK; ++k) v[k] = in[base + (long)k * stride];
+#pragma unroll
+  for (int R = 0; R < NK; ++R) {
+    z64 acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < NK; ++k) acc = cfma(sign < 0 ? w[R * NK + k] : cconj(w[R * NK + k]), v[k], acc);
+    out[base + (long)R * stride] = acc;
+  }
+}
+
+static void dft_pass(long nfield, int nx, int ny, int nz, int axis, int sign, const z64* tw, const z64* in, z64* out,
+                     hipStream_t s) {
+  const int n = axis == 2 ? nz : axis == 1 ? ny : nx;
+  const long nline = nfield * ((long)nx * ny * nz / n);
+  const dim3 lg((unsigned)((nline + 255) / 256));
+  switch (n) {
+#define ADGA_DL(NK)                                                                                                \
+  case NK:                                                                                                        \
+    hipLaunchKernelGGL(k_dft_line<NK>, lg, dim3(256), 0, s, nfield, nx, ny, nz, axis, sign, tw, in, out);        \
+    return;
+    ADGA_DL(2) ADGA_DL(3) ADGA_DL(4) ADGA_DL(5) ADGA_DL(6) ADGA_DL(7) ADGA_DL(8) ADGA_DL(9) ADGA_DL(10)
+    ADGA_DL(11) ADGA_DL(12) ADGA_DL(13) ADGA_DL(14) ADGA_DL(15) ADGA_DL(16) ADGA_DL(17) ADGA_DL(18)
+    ADGA_DL(19) ADGA_DL(20) ADGA_DL(21) ADGA_DL(22) ADGA_DL(23) ADGA_DL(24)
+#undef ADGA_DL
+    default:
+      break;
+  }
+  const long tot = nfield * nx * ny * nz;
+  hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nfield, nx, ny, nz, axis, sign,
+                     tw, in, out);
+}
+
 // Three axis passes: a -> b -> a -> b; the transform ends in b.
 void dft3(const Geom& g, long nfield, int sign, const z64* tw, z64* a, z64* b, hipStream_t s) {
-  long tot = nfield * g.nkp;
-  dim3 grid((unsigned)((tot + 255) / 256));
-  hipLaunchKernelGGL(k_dft_axis, grid, dim3(256), 0, s, nfield, g.nkx, g.nky, g.nkz, 2, sign, tw, a, b);
-  hipLaunchKernelGGL(k_dft_axis, grid, dim3(256), 0, s, nfield, g.nkx, g.nky, g.nkz, 1, sign, tw, b, a);
-  hipLaunchKernelGGL(k_dft_axis, grid, dim3(256), 0, s, nfield, g.nkx, g.nky, g.nkz, 0, sign, tw, a, b);
+  dft_pass(nfield, g.nkx, g.nky, g.nkz, 2, sign, tw, a, b, s);
+  dft_pass(nfield, g.nkx, g.nky, g.nkz, 1, sign, tw, b, a, s);
+  dft_pass(nfield, g.nkx, g.nky, g.nkz, 0, sign, tw, a, b, s);
 }
 
 // G table [w][k][ab] -> fields [w][ab][k]
