@@ -172,7 +172,8 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
         }
         __syncthreads();
         // 16-byte stores: ldm is a multiple of 16 and d0 * n2 is even, so every row segment starts
-        // 16-byte aligned
+        // 16-byte aligned; non-temporal, as M is read again only by the factorisation after the whole
+        // build (20.6 -> 19.2 ms per complex64 build of 8000 matrices at N = 1080, 34.8 -> 33.1 complex128)
         cplx<T>* mb = M + ((long)(2 * (b0 + pb) + ch) * N + r0) * ldm + d0 * N2;
         constexpr int VW = 16 / (int)sizeof(cplx<T>), WV = W / VW;
         static_assert(W % VW == 0, "row segment of whole 16-byte vectors");
@@ -186,7 +187,15 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
             Vec x;
 #pragma unroll
             for (int j = 0; j < VW; ++j) x.v[j] = tile[row][col + j];
-            *reinterpret_cast<Vec*>(dst) = x;
+            if constexpr (sizeof(T) == 4) {
+              typedef float f4 __attribute__((ext_vector_type(4)));
+              const f4 vv = {x.v[0].re, x.v[0].im, x.v[1].re, x.v[1].im};
+              __builtin_nontemporal_store(vv, reinterpret_cast<f4*>(dst));
+            } else {
+              typedef double d2 __attribute__((ext_vector_type(2)));
+              const d2 vv = {x.v[0].re, x.v[0].im};
+              __builtin_nontemporal_store(vv, reinterpret_cast<d2*>(dst));
+            }
           } else {
             for (int j = 0; j < VW; ++j)
               if (col + j < lim) dst[j] = tile[row][col + j];
