@@ -298,10 +298,12 @@ def small_workload(api, synth, cpu=True, procs=16):
     return res
 
 
-def gradient(synth):
+def gradient(synth, cpu=False, procs=16, cpu_points=4):
     """The differentiation workload (3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all 36288
     points): the forward pass (complex128) and the backward pass (forward + tangent-linear along t, t', U)
-    of adga_amd.grad.ChiSigma, wall time each, after one untimed forward pass."""
+    of adga_amd.grad.ChiSigma, wall time each, after one untimed forward pass.  With `cpu`, the forward
+    path of the reference's modules on the same workload beside it (cpu_baseline's method: `procs`
+    single-threaded processes, `cpu_points` points each at omega = 0); the reference has no derivatives."""
     import torch
     from adga_amd import grad
     wl = synth.workload("t2g_k12")
@@ -314,11 +316,22 @@ def gradient(synth):
     loss = chi.real.sum() + sig.real.sum()
     loss.backward()
     t2 = time.perf_counter()
-    return {"workload": "t2g_k12: 3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all points",
-            "points": npts, "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
-            "forward_points_per_s": round(npts / (t1 - t0), 3),
-            "backward_points_per_s": round(npts / (t2 - t1), 3), "directions": 3,
-            "grad": [float(x) for x in theta.grad]}
+    res = {"workload": "t2g_k12: 3-orbital t2g, 12^3 k, 2x40 x 21 frequencies, N = 720, all points",
+           "points": npts, "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+           "forward_points_per_s": round(npts / (t1 - t0), 3),
+           "backward_points_per_s": round(npts / (t2 - t1), 3), "directions": 3,
+           "grad": [float(x) for x in theta.grad]}
+    if cpu:
+        inp = synth.build_inputs(wl)
+        cache = {}
+
+        def chi_cache(w):
+            if w not in cache:
+                cache[w] = (synth.chi_loc(inp, w, "dens"), synth.chi_loc(inp, w, "magn"))
+            return cache[w]
+        res["cpu_forward_reference"] = cpu_baseline(synth, inp, chi_cache, cpu_points, procs)
+        res["gpu_forward_over_cpu"] = round(res["forward_points_per_s"] / res["cpu_forward_reference"]["value"], 1)
+    return res
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -384,7 +397,8 @@ def main():
         print(json.dumps({"shrinking_box": shrinking_box(api, synth)}), flush=True)
         return
     if args.only_gradient:
-        print(json.dumps({"gradient": gradient(synth)}), flush=True)
+        print(json.dumps({"gradient": gradient(synth, cpu=not args.no_cpu_baseline, procs=args.cpu_procs)}),
+              flush=True)
         return
     if args.only_small_workload:
         print(json.dumps({"small_workload": small_workload(api, synth, cpu=not args.no_cpu_baseline,
@@ -474,7 +488,7 @@ def main():
     if not args.no_shrinking_box:
         line["shrinking_box"] = shrinking_box(api, synth)
     if not args.no_gradient:
-        line["gradient"] = gradient(synth)
+        line["gradient"] = gradient(synth, cpu=not args.no_cpu_baseline, procs=args.cpu_procs)
     if not args.no_small_workload:
         line["small_workload"] = small_workload(api, synth, cpu=not args.no_cpu_baseline, procs=args.cpu_procs)
     print(json.dumps(line), flush=True)
