@@ -121,7 +121,7 @@ struct BuildShape {
   static constexpr int DB = NN <= 3 ? (ES == 8 ? 16 : 8) : 2;
   static constexpr int PB0 = 40960 / (DB * N4 * ES);
   static constexpr int PB1 = PB0 < 1 ? 1 : PB0 > 4 ? 4 : PB0;
-  static constexpr int PB = (ES == 8 && NN <= 3 && PB1 > 2) ? 2 : PB1;
+  static constexpr int PB = NN <= 3 ? ((ES == 8 && PB1 > 2) ? 2 : PB1) : (PB1 < (ES == 8 ? 2 : 1) ? PB1 : (ES == 8 ? 2 : 1));
 };
 
 template <typename T, int NN>
@@ -145,84 +145,66 @@ __global__ void __launch_bounds__(256) k_build(Geom g, int npts, int ldm, const 
   }
   __syncthreads();
   const z64* C = ch == 0 ? c_d : c_m;
-  const int dl = t % DB, rl = t / DB;
-  const int d = d0 + dl;
-  if constexpr (NN <= 3) {
-    // outputs go through an LDS tile so that each row segment (nd*n2 complex) is written by
-    // consecutive lanes
-    constexpr int RS = 256 / DB, W = DB * N2;
-    __shared__ cplx<T> tile[RS][W + 1];
-    for (int r0 = 0; r0 < N; r0 += RS) {
-      const int r = r0 + rl;
-      const bool act = dl < nd && r < N;
-      CA cc[N2];
-      const z64* crow = C + (long)(act ? r : 0) * N + (act ? d : 0) * N2;
-#pragma unroll
-      for (int i = 0; i < N2; ++i) cc[i] = ccast<T>(crow[i]);
-      for (int pb = 0; pb < np; ++pb) {
-        if (act) {
-          const CA* sd = sS[pb] + dl * N4;
-#pragma unroll 1
-          for (int sq = 0; sq < N2; ++sq) {
-            CA acc = mk<T>(0, 0);
-#pragma unroll
-            for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
-            tile[rl][dl * N2 + sq] = mk<T>((d * N2 + sq == r ? (T)1 : (T)0) - acc.re, -acc.im);
-          }
-        }
-        __syncthreads();
-        // 16-byte stores: ldm is a multiple of 16 and d0 * n2 is even, so every row segment starts
-        // 16-byte aligned; non-temporal, as M is read again only by the factorisation after the whole
-        // build (20.6 -> 19.2 ms per complex64 build of 8000 matrices at N = 1080, 34.8 -> 33.1 complex128)
-        cplx<T>* mb = M + ((long)(2 * (b0 + pb) + ch) * N + r0) * ldm + d0 * N2;
-        constexpr int VW = 16 / (int)sizeof(cplx<T>), WV = W / VW;
-        static_assert(W % VW == 0, "row segment of whole 16-byte vectors");
-        struct alignas(16) Vec { cplx<T> v[VW]; };
-        const int lim = nd * N2;
-        for (int e = t; e < RS * WV; e += 256) {
-          const int row = e / WV, col = (e % WV) * VW;
-          if (r0 + row >= N) continue;
-          cplx<T>* dst = mb + (long)row * ldm + col;
-          if (col + VW <= lim) {
-            Vec x;
-#pragma unroll
-            for (int j = 0; j < VW; ++j) x.v[j] = tile[row][col + j];
-            if constexpr (sizeof(T) == 4) {
-              typedef float f4 __attribute__((ext_vector_type(4)));
-              const f4 vv = {x.v[0].re, x.v[0].im, x.v[1].re, x.v[1].im};
-              __builtin_nontemporal_store(vv, reinterpret_cast<f4*>(dst));
-            } else {
-              typedef double d2 __attribute__((ext_vector_type(2)));
-              const d2 vv = {x.v[0].re, x.v[0].im};
-              __builtin_nontemporal_store(vv, reinterpret_cast<d2*>(dst));
-            }
-          } else {
-            for (int j = 0; j < VW; ++j)
-              if (col + j < lim) dst[j] = tile[row][col + j];
-          }
-        }
-        __syncthreads();
-      }
-    }
-    return;
-  }
-  if (dl >= nd) return;
-  for (int r = rl; r < N; r += 256 / DB) {
+  // outputs go through an LDS tile so that each row segment (nd*n2 complex) is written by consecutive
+  // lanes.  QT threads share a (row, frequency block) and split its n2 outputs: one for n <= 3, four
+  // for n = 4, 5 (16 or 25 outputs per row and block), whose tile would otherwise span 128 rows
+  // (52 KB of complex64 at n = 5) and leave few workgroups per CU
+  constexpr int QT = NN <= 3 ? 1 : 4, QS = (N2 + QT - 1) / QT;
+  constexpr int RS = 256 / (DB * QT), W = DB * N2;
+  const int q = t % QT, dl = (t / QT) % DB, rl = t / (QT * DB);
+  const int d = d0 + dl, sq0 = q * QS, sq1 = min(N2, sq0 + QS);
+  __shared__ cplx<T> tile[RS][W + 1];
+  for (int r0 = 0; r0 < N; r0 += RS) {
+    const int r = r0 + rl;
+    const bool act = dl < nd && r < N;
     CA cc[N2];
-    const z64* crow = C + (long)r * N + d * N2;
+    const z64* crow = C + (long)(act ? r : 0) * N + (act ? d : 0) * N2;
 #pragma unroll
     for (int i = 0; i < N2; ++i) cc[i] = ccast<T>(crow[i]);
     for (int pb = 0; pb < np; ++pb) {
-      const CA* sd = sS[pb] + dl * N4;
-      cplx<T>* mrow = M + ((long)(2 * (b0 + pb) + ch) * N + r) * ldm + d * N2;
-      // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
+      if (act) {
+        const CA* sd = sS[pb] + dl * N4;
+        // not unrolled: S_d stays in LDS (hoisting its n2^2 values into registers costs occupancy)
 #pragma unroll 1
-      for (int sq = 0; sq < N2; ++sq) {
-        CA acc = mk<T>(0, 0);
+        for (int sq = sq0; sq < sq1; ++sq) {
+          CA acc = mk<T>(0, 0);
 #pragma unroll
-        for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
-        mrow[sq] = mk<T>((d * N2 + sq == r ? (T)1 : (T)0) - acc.re, -acc.im);
+          for (int i = 0; i < N2; ++i) acc = cfma(sd[sq * N2 + i], cc[i], acc);
+          tile[rl][dl * N2 + sq] = mk<T>((d * N2 + sq == r ? (T)1 : (T)0) - acc.re, -acc.im);
+        }
       }
+      __syncthreads();
+      // 16-byte stores: ldm is a multiple of 16 and d0 * n2 is even, so every row segment starts
+      // 16-byte aligned; non-temporal, as M is read again only by the factorisation after the whole
+      // build (20.6 -> 19.2 ms per complex64 build of 8000 matrices at N = 1080, 34.8 -> 33.1 complex128)
+      cplx<T>* mb = M + ((long)(2 * (b0 + pb) + ch) * N + r0) * ldm + d0 * N2;
+      constexpr int VW = 16 / (int)sizeof(cplx<T>), WV = W / VW;
+      static_assert(W % VW == 0, "row segment of whole 16-byte vectors");
+      struct alignas(16) Vec { cplx<T> v[VW]; };
+      const int lim = nd * N2;
+      for (int e = t; e < RS * WV; e += 256) {
+        const int row = e / WV, col = (e % WV) * VW;
+        if (r0 + row >= N) continue;
+        cplx<T>* dst = mb + (long)row * ldm + col;
+        if (col + VW <= lim) {
+          Vec x;
+#pragma unroll
+          for (int j = 0; j < VW; ++j) x.v[j] = tile[row][col + j];
+          if constexpr (sizeof(T) == 4) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 vv = {x.v[0].re, x.v[0].im, x.v[1].re, x.v[1].im};
+            __builtin_nontemporal_store(vv, reinterpret_cast<f4*>(dst));
+          } else {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            const d2 vv = {x.v[0].re, x.v[0].im};
+            __builtin_nontemporal_store(vv, reinterpret_cast<d2*>(dst));
+          }
+        } else {
+          for (int j = 0; j < VW; ++j)
+            if (col + j < lim) dst[j] = tile[row][col + j];
+        }
+      }
+      __syncthreads();
     }
   }
 }
