@@ -390,3 +390,25 @@ def test_gpu_repeated_q_in_a_batch():
     for k in ("chi_d", "chi_m", "bubble_nl"):
         assert np.array_equal(res[1][k][:, :, 2], res[1][k][:, :, 0]), k
         assert np.array_equal(res[1][k][:, :, 4], res[1][k][:, :, 1]), k
+
+
+def test_gpu_anisotropic_grid_with_a_long_axis():
+    """A 28 x 3 x 2 k grid (q on the same grid) with a stronger vertex (noise 0.05): the k-grid
+    transforms of the bubble and EOM run their x passes through the per-output kernel (28 points, beyond
+    the per-line kernel's largest line) and the y and z passes per line; chi, bubble_nl and the
+    self-energies against the oracle in both precisions, and against the direct sums."""
+    import adga_oracle
+    api = _api()
+    wl = synth.workload("sc1_k8", nk=(28, 3, 2), noise=0.05)
+    inp = synth.build_inputs(wl)
+    pts = [(0, 0), (0, 37), (0, 90), (2, 101), (-3, 167), (-3, 5)]
+    cl = lambda w, c: synth.chi_loc(inp, w, c)  # noqa: E731
+    print("anisotropic grid 28 x 3 x 2", flush=True)
+    ref = adga_oracle.HotPath(inp).run(pts, cl)
+    for precision, modes in (("fp64", (1, 1)), ("refined", (1, 1)), ("fp64", (0, 0))):
+        with api.Context(inp, bubble=modes[0], eom=modes[1]) as ctx:
+            out = api.run_loop(ctx, pts, cl, precision)
+        assert (out["status"] == 0).all()
+        for k in ("chi_d", "chi_m", "bubble_nl", "sigma_nl", "sigma_dmft"):
+            e = rel_err(out[k], ref[k])
+            assert e < 1e-10, (precision, modes, k, e)
