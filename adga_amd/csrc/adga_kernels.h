@@ -100,9 +100,9 @@ void launch_vq_add(const Geom& g, int npts, const VqTerm& v, const z64* Y, z64* 
 void launch_eom_fock(const Geom& g, int npts, const int* qpos, const int* qidx, const int* keom, const z64* vq,
                      const z64* nfock, z64* shf, hipStream_t s);
 void launch_eom_hartree_v(const Geom& g, const z64* v, const z64* ndmft, z64* shf, hipStream_t s);
-void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
-                    const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
-                    z64* chi_out, z64* bub_out, hipStream_t s);
+// chi_ch(q) and bubble_nl of a batch; eta = the batch's solutions Y (2 npts matrices, eta's layout)
+void launch_chi(const Geom& g, int npts, const z64* eta, const z64* chi0q, const z64* chi0w, const z64* gamd,
+                const z64* gamm, const z64* bubloc, z64* chi_out, z64* bub_out, hipStream_t s);
 // zeroes the per-point blocks (`len` complex values each) of the points whose flag is zero
 void launch_zero_points(int npts, long len, const int* keep, z64* a, hipStream_t s);
 // dst[i*len .. ) = src[sel[i]*len .. ) for i < k (gather) or the reverse (scatter)
@@ -140,7 +140,6 @@ void launch_tangent_rhs(const Geom& g, int npts, const z64* S, const z64* dS, co
 void launch_dchi(const Geom& g, int npts, const z64* chi0q, const z64* dchi0q, const z64* chi0w, const z64* dchi0w,
                  const z64* gamd, const z64* gamm, const z64* dgamd, const z64* dgamm, const z64* Y, const z64* dY,
                  z64* dchi_out, z64* dbub_out, hipStream_t s);
-void launch_eta(const Geom& g, int nmat, const z64* Y, z64* eta, hipStream_t s);
 // k-grid convolutions (kconv.hip)
 void dft3(const Geom& g, long nfield, int sign, const z64* tw, z64* a, z64* b, hipStream_t s);
 void launch_ghat(const Geom& g, const z64* tw, const z64* gt, z64* tmp, z64* ghat, hipStream_t s);

@@ -11,7 +11,7 @@ using namespace adga_host;
 namespace {
 
 void free_batch(adga_ctx* c) {
-  dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R); dfree(c->eta);
+  dfree(c->chi0q); dfree(c->S); dfree(c->gamq); dfree(c->Bp); dfree(c->Y); dfree(c->Z); dfree(c->R);
   dfree(c->mt); dfree(c->chi_out); dfree(c->bub_out); dfree(c->M); dfree(c->inv); dfree(c->rel);
   dfree(c->piv); dfree(c->info); dfree(c->qidx); dfree(c->act); dfree(c->keep);
   c->cap = 0;
@@ -38,7 +38,6 @@ void ensure_batch(adga_ctx* c, int nb) {
   dalloc(&c->Y, 2 * nb * pmat);
   dalloc(&c->Z, 2 * nb * pmat);
   dalloc(&c->R, 2 * nb * pmat);
-  dalloc(&c->eta, 2 * nb * pmat);
   dalloc(&c->mt, nb * pmat);
   dalloc(&c->chi_out, 2 * nb * (size_t)g.n4);
   dalloc(&c->bub_out, nb * (size_t)g.n4);
@@ -715,8 +714,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
         }
       }
       CK(hipEventRecord(ev[4], s));
-      adga::launch_eta_chi(g, nb, c->Y, c->opgd, c->opgm, c->eta, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc,
-                           c->chi_out, c->bub_out, s);
+      adga::launch_chi(g, nb, c->Y, c->chi0q, c->chi0w, c->gamd, c->gamm, c->bubloc, c->chi_out, c->bub_out, s);
       if (c->bubble_range >= 1) {
         // the large range's frequencies outside the box enter bubble_nl (main.F90:644-663, debug option
         // "Bubble"): all q of the omega as one convolution, kept for the omega's next batches
@@ -736,7 +734,7 @@ int run_points_impl(adga_ctx* c, int npts, const int* iq, int precision, double*
       }
       CK(hipEventRecord(ev[5], s));
       if (c->do_eom) {
-        adga::launch_mtot(g, nb, c->u, c->ut, c->eta, c->gamq, c->mt, s, c->qpos, c->vq, c->gamd);
+        adga::launch_mtot(g, nb, c->u, c->ut, c->Y, c->gamq, c->mt, s, c->qpos, c->vq, c->gamd);  // eta = Y
         if (nfailed > 0) {
           CK(hipMemcpyAsync(c->keep, keep.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
           adga::launch_zero_points(nb, (long)n2 * g.N, c->keep, c->mt, s);

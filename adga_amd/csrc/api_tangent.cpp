@@ -14,7 +14,7 @@ namespace {
 
 void ensure_tangent_batch(adga_ctx* c, int nb) {
   if (nb <= c->t_cap) return;
-  for (z64** p : {&c->t_dchi0q, &c->t_dS, &c->t_Z1, &c->t_Z2, &c->t_B, &c->t_dY, &c->t_deta, &c->t_dgamq, &c->t_dmt,
+  for (z64** p : {&c->t_dchi0q, &c->t_dS, &c->t_Z1, &c->t_Z2, &c->t_B, &c->t_dY, &c->t_dgamq, &c->t_dmt,
                   &c->t_dchi, &c->t_dbub})
     dfree(*p);
   const Geom& g = c->g;  // the largest box
@@ -25,7 +25,6 @@ void ensure_tangent_batch(adga_ctx* c, int nb) {
   dalloc(&c->t_Z2, 2 * nb * pmat);
   dalloc(&c->t_B, 2 * nb * pmat);
   dalloc(&c->t_dY, 2 * nb * pmat);
-  dalloc(&c->t_deta, 2 * nb * pmat);
   dalloc(&c->t_dgamq, 2 * nb * pmat);
   dalloc(&c->t_dmt, nb * pmat);
   dalloc(&c->t_dchi, (size_t)c->ndir * 2 * nb * g.n4);
@@ -43,7 +42,7 @@ namespace adga_host {
 
 void free_tangent(adga_ctx* c) {
   for (z64** p : {&c->dgt, &c->dgiw, &c->du, &c->dut, &c->t_dw, &c->t_dwinv, &c->t_dcd, &c->t_dcm, &c->t_dgamd,
-                  &c->t_dgamm, &c->t_dchi0q, &c->t_dS, &c->t_Z1, &c->t_Z2, &c->t_B, &c->t_dY, &c->t_deta, &c->t_dgamq,
+                  &c->t_dgamm, &c->t_dchi0q, &c->t_dS, &c->t_Z1, &c->t_Z2, &c->t_B, &c->t_dY, &c->t_dgamq,
                   &c->t_dmt, &c->t_dchi, &c->t_dbub, &c->t_sig, &c->t_macc, &c->t_dghat, &c->t_mwork,
                   &c->t_dbub_all, &c->t_bub_p})
     dfree(*p);
@@ -105,13 +104,12 @@ void tangent_batch(adga_ctx* c, int nb, int p0, int npts, const std::vector<int>
     if (!c->do_eom) continue;
     // d m_tot = m_tot(U, U~; d eta, d gamma^q) + m_tot(dU, dU~; eta, gamma^q), with
     // d gamma^q_d = sum_nu (dS C_d + S dC_d)[nu rows]
-    adga::launch_eta(g, nmat, c->t_dY, c->t_deta, s);
     adga::launch_gamq_z(g, nb, c->t_dS, c->t_Z1, s);
     adga::resid_gemm(g, nb, 0, 1, c->cd, c->cd, c->t_Z1, c->t_dgamq, nullptr, nullptr, s);
     adga::launch_gamq_z(g, nb, c->S, c->t_Z1, s);
     adga::resid_gemm(g, nb, 0, 1, dcd, dcd, c->t_Z1, c->t_dgamq, nullptr, nullptr, s, true);
-    adga::launch_mtot(g, nb, c->u, c->ut, c->t_deta, c->t_dgamq, c->t_dmt, s);
-    adga::launch_mtot(g, nb, c->du + (size_t)dir * G.n4, c->dut + (size_t)dir * G.n4, c->eta, c->gamq, c->t_dmt, s,
+    adga::launch_mtot(g, nb, c->u, c->ut, c->t_dY, c->t_dgamq, c->t_dmt, s);  // d eta = dY
+    adga::launch_mtot(g, nb, c->du + (size_t)dir * G.n4, c->dut + (size_t)dir * G.n4, c->Y, c->gamq, c->t_dmt, s,
                       nullptr, nullptr, nullptr, true);
     if (nfailed > 0) adga::launch_zero_points(nb, (long)g.n2 * g.N, c->keep, c->t_dmt, s);  // keep flags on the device
     // d Sigma_nl = sum_q [d m_tot . G(k - q) + m_tot . dG(k - q)]: with the EOM as k-grid convolutions,

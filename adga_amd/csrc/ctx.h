@@ -77,7 +77,7 @@ struct adga_ctx {
   z64 *chiloc_d = nullptr, *chiloc_m = nullptr, *bubloc = nullptr;
   // batch workspace
   int cap = 0;
-  z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr, *eta = nullptr,
+  z64 *chi0q = nullptr, *S = nullptr, *gamq = nullptr, *Bp = nullptr, *Y = nullptr, *Z = nullptr, *R = nullptr,
       *mt = nullptr, *chi_out = nullptr, *bub_out = nullptr;
   void* M = nullptr;
   void* inv = nullptr;  // per-matrix panel inverses of the LU
@@ -119,7 +119,7 @@ struct adga_ctx {
   // per-batch derivative workspace and the derivative of Sigma_nl per direction
   int t_cap = 0;
   z64 *t_dchi0q = nullptr, *t_dS = nullptr, *t_Z1 = nullptr, *t_Z2 = nullptr, *t_B = nullptr, *t_dY = nullptr,
-      *t_deta = nullptr, *t_dgamq = nullptr, *t_dmt = nullptr, *t_dchi = nullptr, *t_dbub = nullptr;
+      *t_dgamq = nullptr, *t_dmt = nullptr, *t_dchi = nullptr, *t_dbub = nullptr;
   z64* t_sig = nullptr;
   // EOM derivatives by k-grid convolution (eom mode 1): dm per direction on the q grid, the transform of
   // each direction's dG table, scratch

@@ -3,18 +3,9 @@
 
 namespace adga {
 
-// eta_m(s, x) = Y[m][x][s]: Y = eta^T solves A^T eta^T = B (main.F90:740-757).  Output (n2, N) Fortran.
-__global__ void k_eta(Geom g, int nmat, const z64* __restrict__ Y, const z64* __restrict__ opg_d,
-                      const z64* __restrict__ opg_m, z64* __restrict__ eta) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long per = (long)g.n2 * g.N;
-  if (idx >= per * nmat) return;
-  int m = (int)(idx / per);
-  long e = idx % per;
-  int s = (int)(e % g.n2);
-  long x = e / g.n2;
-  eta[idx] = Y[((long)m * g.N + x) * g.n2 + s];
-}
+// eta_m(s, x) = Y[m][x][s]: Y = eta^T solves A^T eta^T = B (main.F90:740-757), and the solutions' layout
+// (one N x n2 row-major block per matrix) is eta's Fortran (n2, N) layout, so the kernels below read
+// eta from Y itself.
 
 // chi_ch(q) (non-local part) = beta^-2 gamma^w_ch . (chi0^q - chi0^w) + beta^-2 eta_ch . chi0^q, where
 // (X . Y)(s, s2) = sum_{d, i3} X(s, d*n2 + i3) Y(i3, s2, d)  (susc_module.f90:34-56, main.F90:666-769);
@@ -223,11 +214,8 @@ __global__ void k_eom_static(Geom g, const z64* __restrict__ u, const z64* __res
   sdmft[o] = cadd(sdmft[o], acc);
 }
 
-void launch_eta_chi(const Geom& g, int npts, const z64* Y, const z64* opgd, const z64* opgm, z64* eta,
-                    const z64* chi0q, const z64* chi0w, const z64* gamd, const z64* gamm, const z64* bubloc,
-                    z64* chi_out, z64* bub_out, hipStream_t s) {
-  long tot = (long)2 * npts * g.n2 * g.N;
-  hipLaunchKernelGGL(k_eta, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, 2 * npts, Y, opgd, opgm, eta);
+void launch_chi(const Geom& g, int npts, const z64* eta, const z64* chi0q, const z64* chi0w, const z64* gamd,
+                const z64* gamm, const z64* bubloc, z64* chi_out, z64* bub_out, hipStream_t s) {
   if (g.n <= 3) {
 #define ADGA_CHI_WG(NN)                                                                                          \
   if (g.n == NN) {                                                                                              \

@@ -275,19 +275,4 @@ void launch_dchi(const Geom& g, int npts, const z64* chi0q, const z64* dchi0q, c
                      dchi0w, gamd, gamm, dgamd, dgamm, Y, dY, dchi_out, dbub_out);
 }
 
-// eta_m(s, x) = Y[m][x][s] for every matrix (the solution's transpose, Fortran (n2, N) per matrix)
-__global__ void k_eta_only(Geom g, int nmat, const z64* __restrict__ Y, z64* __restrict__ eta) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long per = (long)g.n2 * g.N;
-  if (idx >= per * nmat) return;
-  const int m = (int)(idx / per);
-  const long e = idx % per;
-  eta[idx] = Y[((long)m * g.N + e / g.n2) * g.n2 + e % g.n2];
-}
-
-void launch_eta(const Geom& g, int nmat, const z64* Y, z64* eta, hipStream_t s) {
-  const long tot = (long)nmat * g.n2 * g.N;
-  hipLaunchKernelGGL(k_eta_only, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, nmat, Y, eta);
-}
-
 }  // namespace adga
